@@ -1,0 +1,182 @@
+/*
+ * zfv.h — C-ABI of libzfv_hip.so, the MI355X (gfx950) rate-of-change engine.
+ *
+ * The reference (ZisaFVM) exposes the hot path through two C++ plugin points:
+ *
+ *   RateOfChange::compute(AllVariables &tendency, const AllVariables &state, double t)
+ *       include/zisa/ode/rate_of_change.hpp:37-39
+ *   TimeIntegration::compute_step(shared_ptr<AllVariables> u0, double t, double dt)
+ *       include/zisa/ode/time_integration.hpp:42-44
+ *
+ * plus the per-step CFL hook (include/zisa/model/cfl_condition.hpp:17) and the
+ * per-stage boundary hook BoundaryCondition::apply
+ * (include/zisa/boundary/boundary_condition.hpp:17).  The entry points below
+ * replace exactly those, on plain pointers and sizes.  The C++ adapters in
+ * zisafvm_amd/csrc/adapters.hpp wrap them back into the reference's class
+ * interfaces.
+ *
+ * Conventions
+ *   - State in and out is the reference's AllVariables::cvars layout:
+ *     n_cells x 5 row-major FP64 (rho, rho u, rho v, rho w, E); cell order is the
+ *     caller's (src/zisa/model/all_variables.cpp:25-30).  Internally the
+ *     library renumbers cells along a Hilbert curve and stores structure-of-
+ *     arrays; the boundary transposes.
+ *   - Every function returns 0 on success and a negative code on failure;
+ *     zfv_last_error() gives the message (thread-local).  No exception crosses
+ *     the boundary.  (The reference aborts via LOG_ERR instead.)
+ *   - A handle owns one HIP stream; use a handle from one host thread at a time.
+ */
+#ifndef ZFV_H
+#define ZFV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZFV_MAX_STENCILS 5
+
+enum { ZFV_CWENO_AO = 0, ZFV_WENO_AO = 1 };
+enum { ZFV_FLUX_HLLC = 0, ZFV_FLUX_RUSANOV = 1 };
+enum { ZFV_EQ_NONE = 0, ZFV_EQ_ISENTROPIC = 1, ZFV_EQ_ISOTHERMAL = 2 };
+enum { ZFV_GRAVITY_NONE = 0, ZFV_GRAVITY_AXIAL = 1, ZFV_GRAVITY_RADIAL = 2, ZFV_GRAVITY_POLYTROPE = 3 };
+
+enum {
+  ZFV_OK = 0,
+  ZFV_ERR_INVALID = -1,   /* bad argument / unsupported configuration */
+  ZFV_ERR_HIP = -2,       /* HIP runtime error */
+  ZFV_ERR_NUMERIC = -3,   /* implausible state (rho <= 0, E_int <= 0, NaN) */
+  ZFV_ERR_NO_DEVICE = -4, /* no gfx950 device visible */
+  ZFV_ERR_MEMORY = -5     /* admission / allocation failure */
+};
+
+/* The reference's JSON config keys for the path, flattened:
+ * reconstruction.{mode,orders,biases,overfit_factors,linear_weights,
+ * smoothness_indicator.{epsilon,exponent},steps_per_recompute,recompute_threshold},
+ * well-balancing.mode, euler.{eos,gravity}, quadrature.{edge,volume,moments},
+ * flux-bc.mode (include/zisa/experiments/euler_experiment_impl.hpp:83-92,411-440;
+ * src/zisa/experiments/numerical_experiment.cpp:193-214). */
+typedef struct zfv_params {
+  int32_t n_dims;                 /* 2 (triangles) or 3 (tetrahedra) */
+  int32_t n_stencils;             /* <= ZFV_MAX_STENCILS */
+  int32_t orders[ZFV_MAX_STENCILS];
+  int32_t biases[ZFV_MAX_STENCILS];  /* 0 = central "c", 1 = one-sided "b" */
+  double overfit[ZFV_MAX_STENCILS];
+  double linear_weights[ZFV_MAX_STENCILS];
+  double epsilon;                 /* smoothness_indicator.epsilon */
+  double exponent;                /* smoothness_indicator.exponent */
+  int32_t reconstruction;         /* ZFV_CWENO_AO / ZFV_WENO_AO */
+  int32_t flux;                   /* ZFV_FLUX_* */
+  int32_t equilibrium;            /* ZFV_EQ_* (well-balancing.mode) */
+  int32_t gravity;                /* ZFV_GRAVITY_* */
+  double gravity_g;               /* constant gravity strength */
+  double gravity_axis[3];         /* axial alignment axis */
+  double polytrope_rhoC, polytrope_K, polytrope_G;
+  double gamma;                   /* ideal gas */
+  double gas_constant;
+  int32_t edge_deg, volume_deg, moments_deg;  /* quadrature degrees, -1 = max */
+  int32_t flux_bc;                /* 0 none, 1 constant-extrapolation FluxBC */
+  int32_t steps_per_recompute;    /* must be 1: equilibria are re-solved every call */
+  int32_t reserved;
+  double recompute_threshold;
+} zfv_params;
+
+/* A simplex mesh as the reference's .msh.h5 carries it (vertex_indices), plus
+ * per-cell corner coordinates.  Corners are per cell so that periodic meshes
+ * can carry unwrapped coordinates; neighbours are found through shared vertex
+ * ids (src/zisa/grid/grid.cpp:486-540). */
+typedef struct zfv_mesh {
+  int32_t n_dims;
+  int32_t n_cells;
+  const int32_t *vertex_ids;  /* n_cells * (n_dims+1) */
+  const double *corners;      /* n_cells * (n_dims+1) * 3 */
+  const uint8_t *ghost_mask;  /* n_cells (nonzero = ghost cell, FrozenBC), or NULL */
+} zfv_mesh;
+
+typedef struct zfv_handle zfv_handle;
+
+/* Geometry, stencils, LSQ pseudo-inverses and the device image of one mesh.
+ * Replaces the setup half of make_reconstruction_array + FluxLoop ctor
+ * (include/zisa/reconstruction/global_reconstruction_decl.hpp:105-147,
+ *  include/zisa/fvm_loops/flux_loop.hpp:29-90). */
+int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out);
+int zfv_destroy(zfv_handle *h);
+
+/* AllVariables::cvars in / out (host memory, caller's cell order). */
+int zfv_set_state(zfv_handle *h, const double *cvars);
+int zfv_get_state(zfv_handle *h, double *cvars);
+
+/* FrozenBC steady state (src/zisa/boundary/frozen_boundary_condition.cpp:11-37). */
+int zfv_set_steady_state(zfv_handle *h, const double *cvars);
+
+/* RateOfChange::compute on host buffers: dudt += L(u).  The sum of terms is
+ * FluxLoop + GravitySourceLoop + FluxBC, in that order
+ * (src/zisa/ode/rate_of_change.cpp:21-27). */
+int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t);
+
+/* TimeIntegration::compute_step for SSP3 on the device-resident state, with the
+ * frozen BC after every stage sum (src/zisa/ode/runge_kutta.cpp:87-112). */
+int zfv_step(zfv_handle *h, double t, double dt);
+
+/* n_steps fixed-dt SSP3 steps replayed from one captured HIP graph. */
+int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt);
+
+/* LocalCFL::operator() on the device-resident state
+ * (include/zisa/model/local_cfl_condition_impl.hpp:25-40). */
+int zfv_cfl_dt(zfv_handle *h, double cfl_number, double *dt);
+
+int zfv_synchronize(zfv_handle *h);
+
+typedef struct zfv_info {
+  int64_t n_cells;
+  int64_t n_faces;            /* interior faces evaluated by the face kernel */
+  int64_t device_bytes;       /* HBM held by the handle */
+  double bytes_per_cell_stage;/* algorithmic bytes, all kernels of one stage */
+  double recon_bytes_per_cell;/* algorithmic bytes of the reconstruction kernel */
+  int32_t max_stencil_size;   /* largest combined stencil */
+  int32_t n_points_face;      /* quadrature points per face */
+} zfv_info;
+int zfv_get_info(zfv_handle *h, zfv_info *info);
+
+/* Combined stencil (local2global) of cell i, caller's numbering, for parity
+ * checks against an independent stencil selection.  Writes up to cap ids; the
+ * per-stencil member lists follow in `members` as
+ * [n_stencils][order, size, id_0 .. id_{size-1}] flattened.  Returns the
+ * number of ints written to `members` (or < 0 on error). */
+int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_t *l2g_size,
+                     int32_t *members, int32_t members_cap);
+
+/* Per-kernel timing of the last zfv_steps call, milliseconds summed over the
+ * launches of each kernel (HIP events on the handle's stream). */
+typedef struct zfv_kernel_times {
+  double reconstruct_ms, face_flux_ms, update_ms;
+  int32_t n_launches;
+} zfv_kernel_times;
+int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out);
+
+const char *zfv_last_error(void);
+
+/* ---- Synthetic workloads (mesh generation is not in the reference, which loads
+ * meshes from files; these produce the seeded jittered meshes of the
+ * benchmark configurations). */
+typedef struct zfv_mesh_buf zfv_mesh_buf;
+/* 2D: n x n jittered lattice on [0,1]^2, each square split along its Delaunay
+ * diagonal (2 n^2 triangles).  periodic_x / periodic_y wrap the lattice; a
+ * non-periodic direction gets `ghost_layers` layers of ghost cells at each end
+ * (cells whose centroid lies outside [0,1] after the domain is extended). */
+int zfv_mesh_lattice2d(int32_t n, double jitter, uint64_t seed, int32_t periodic_x,
+                       int32_t periodic_y, int32_t ghost_layers, zfv_mesh_buf **out);
+/* 3D: jittered cubic lattice of the ball of radius `radius` (Kuhn subdivision,
+ * 6 tets per cube, cells kept when their centroid is inside the ball); cells
+ * whose centroid radius exceeds ghost_radius are ghosts. */
+int zfv_mesh_ball3d(int32_t n, double radius, double ghost_radius, double jitter, uint64_t seed,
+                    zfv_mesh_buf **out);
+int zfv_mesh_view(const zfv_mesh_buf *buf, zfv_mesh *view);
+int zfv_mesh_free(zfv_mesh_buf *buf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZFV_H */

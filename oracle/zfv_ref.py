@@ -1,0 +1,182 @@
+"""ctypes binding of the CPU oracle (oracle/build/libzfv_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product (zisafvm_amd)
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libzfv_ref.so")
+
+
+def build(force: bool = False) -> str:
+    cmd = ["make", "-C", _HERE] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    from zisafvm_amd import CMesh, Params  # struct layouts only (include/zfv.h)
+
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    sig = {
+        "zfv_ref_create": (c_int, [ctypes.POINTER(CMesh), ctypes.POINTER(Params), ctypes.POINTER(vp)]),
+        "zfv_ref_destroy": (c_int, [vp]),
+        "zfv_ref_rate": (c_int, [vp, dp, dp, c_double]),
+        "zfv_ref_step": (c_int, [vp, dp, c_double, c_double]),
+        "zfv_ref_cfl_dt": (c_int, [vp, dp, c_double, dp]),
+        "zfv_ref_set_steady_state": (c_int, [vp, dp]),
+        "zfv_ref_set_threads": (c_int, [vp, c_int]),
+        "zfv_ref_eval": (c_int, [vp, c_int, dp, dp]),
+        "zfv_ref_geometry": (c_int, [vp, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]),
+        "zfv_ref_counts": (c_int, [vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+        "zfv_ref_get_stencils": (c_int, [vp, c_int, ip, c_int, ip, ip, c_int]),
+        "zfv_ref_ldlt_solve": (c_int, [c_int, dp, c_int, dp]),
+        "zfv_ref_rank": (c_int, [c_int, c_int, dp]),
+        "zfv_ref_hllc": (c_int, [c_double, dp, dp, dp]),
+        "zfv_ref_rusanov": (c_int, [c_double, dp, dp, dp]),
+        "zfv_ref_rule": (c_int, [c_int, c_int, dp, dp, c_int]),
+        "zfv_ref_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + (load().zfv_ref_last_error() or b"").decode())
+
+
+class Oracle:
+    """The reference's RateOfChange / SSP3 / LocalCFL restated on the CPU."""
+
+    def __init__(self, mesh, params, threads: int = 1):
+        self.lib = load()
+        self.mesh = mesh
+        self._cm = mesh.c_struct()
+        h = ctypes.c_void_p()
+        _check(self.lib.zfv_ref_create(ctypes.byref(self._cm), ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+        self.lib.zfv_ref_set_threads(h, threads)
+
+    def close(self):
+        if self.h:
+            self.lib.zfv_ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_threads(self, n: int):
+        self.lib.zfv_ref_set_threads(self.h, n)
+
+    def rate(self, u, t=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u)
+        _check(self.lib.zfv_ref_rate(self.h, _dp(u), _dp(out), t))
+        return out
+
+    def step(self, u, t, dt):
+        u = np.ascontiguousarray(u, dtype=np.float64).copy()
+        _check(self.lib.zfv_ref_step(self.h, _dp(u), t, dt))
+        return u
+
+    def cfl_dt(self, u, cfl):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        dt = ctypes.c_double()
+        _check(self.lib.zfv_ref_cfl_dt(self.h, _dp(u), cfl, ctypes.byref(dt)))
+        return dt.value
+
+    def set_steady_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._steady = u
+        _check(self.lib.zfv_ref_set_steady_state(self.h, _dp(u)))
+
+    def eval(self, i, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(5)
+        _check(self.lib.zfv_ref_eval(self.h, i, _dp(x), _dp(out)))
+        return out
+
+    def geometry(self):
+        n = self.mesh.n_cells
+        vol, ctr, inr, ln = np.zeros(n), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+        gh = np.zeros(n, dtype=np.uint8)
+        self.lib.zfv_ref_geometry(self.h, _dp(vol), _dp(ctr), _dp(inr), _dp(ln),
+                                  gh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return dict(volumes=vol, centers=ctr, inradii=inr, lengths=ln, ghost=gh)
+
+    def counts(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.lib.zfv_ref_counts(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def stencils(self, i):
+        l2g = (ctypes.c_int32 * 512)()
+        size = ctypes.c_int32()
+        mem = (ctypes.c_int32 * 1024)()
+        w = self.lib.zfv_ref_get_stencils(self.h, i, l2g, 512, ctypes.byref(size), mem, 1024)
+        assert w >= 0
+        return list(l2g[: size.value]), list(mem[:w])
+
+
+def ldlt_solve(M, b):
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64).copy()
+    n = M.shape[0]
+    nrhs = 1 if b.ndim == 1 else b.shape[1]
+    _check(load().zfv_ref_ldlt_solve(n, _dp(M), nrhs, _dp(b)))
+    return b
+
+
+def rank(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    return load().zfv_ref_rank(A.shape[0], A.shape[1], _dp(A))
+
+
+def hllc(gamma, uL, uR):
+    f = np.zeros(5)
+    load().zfv_ref_hllc(gamma, _dp(np.ascontiguousarray(uL, float)), _dp(np.ascontiguousarray(uR, float)), _dp(f))
+    return f
+
+
+def rusanov(gamma, uL, uR):
+    f = np.zeros(5)
+    load().zfv_ref_rusanov(gamma, _dp(np.ascontiguousarray(uL, float)), _dp(np.ascontiguousarray(uR, float)), _dp(f))
+    return f
+
+
+def rule(kind, deg):
+    w = np.zeros(32)
+    x = np.zeros(32 * 4)
+    n = load().zfv_ref_rule(kind, deg, _dp(w), _dp(x), 32)
+    return w[:n].copy(), x[: 4 * n].reshape(n, 4).copy()

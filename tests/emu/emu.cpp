@@ -1,0 +1,174 @@
+// TEST-ONLY CPU emulation of the HIP kernels.
+//
+// Runs the product's own setup (zisafvm_amd/csrc/setup.cpp) and the same
+// per-cell / per-face functions the gfx950 kernels call
+// (zisafvm_amd/csrc/stage_math.hpp) in host loops, so the CPU test suite can
+// check the product's numerics against the oracle without a GPU.  The
+// product library never loads this; GPU tests call libzfv_hip.so.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../zisafvm_amd/csrc/common.hpp"
+#include "../../zisafvm_amd/csrc/setup.hpp"
+#include "../../zisafvm_amd/csrc/stage_math.hpp"
+
+using namespace zfv;
+
+namespace {
+thread_local std::string g_err;
+
+struct Emu {
+  HostImage img;
+  Img m{};
+  std::vector<double> U, fpv, G, src;
+};
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+void run(Emu &e, const double *U, double *L) {
+  constexpr int NV = DIM + 2;
+  const int64_t n = e.m.n;
+  for (int64_t i = 0; i < n; ++i)
+    reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(e.m, i, U, e.fpv.data(), e.src.empty() ? nullptr : e.src.data());
+  for (int64_t f = 0; f < e.m.ne; ++f) face_flux<DIM, Q>(e.m, f, e.fpv.data(), e.G.data());
+  for (int64_t i = 0; i < n; ++i) {
+    double l[NV];
+    cell_rate<DIM>(e.m, i, U, e.G.data(), e.src.empty() ? nullptr : e.src.data(), l);
+    for (int v = 0; v < NV; ++v) L[v * n + i] = l[v];
+  }
+}
+
+void dispatch(Emu &e, const double *U, double *L) {
+  const Shape &s = e.img.s;
+#define EMU_SHAPE(D, DOF, RC_, NB_, RB_, Q_) \
+  if (s.dim == D && s.dofc == DOF && s.rc == RC_ && s.nb == NB_ && s.rb == RB_ && s.q == Q_) \
+    return run<D, DOF, RC_, NB_, RB_, Q_>(e, U, L);
+  EMU_SHAPE(2, 6, 10, 3, 3, 2)
+  EMU_SHAPE(2, 6, 10, 3, 3, 3)
+  EMU_SHAPE(2, 10, 18, 3, 3, 3)
+  EMU_SHAPE(2, 10, 18, 3, 3, 2)
+  EMU_SHAPE(2, 3, 4, 3, 3, 1)
+  EMU_SHAPE(2, 3, 4, 3, 3, 2)
+  EMU_SHAPE(3, 10, 18, 4, 4, 3)
+  EMU_SHAPE(3, 10, 18, 4, 4, 4)
+  EMU_SHAPE(3, 10, 18, 3, 4, 3)
+#undef EMU_SHAPE
+  throw ZfvError(ZFV_ERR_INVALID, "emulation: shape not instantiated");
+}
+}  // namespace
+
+extern "C" {
+
+const char *zfv_emu_last_error(void) { return g_err.c_str(); }
+
+int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
+  try {
+    auto e = std::make_unique<Emu>();
+    build_image(*mesh, *p, e->img);
+    const HostImage &im = e->img;
+    Img &m = e->m;
+    m.n = im.n;
+    m.ne = im.ne;
+    m.P = im.P.data();
+    m.rows = im.rows.data();
+    m.crn = im.crn.data();
+    m.xc = im.xc.data();
+    m.len = im.len.data();
+    m.vol = im.vol.data();
+    m.inr = im.inr.data();
+    m.mom = im.mom.data();
+    m.meta = im.meta.data();
+    m.cface = im.cface.data();
+    m.fperm = im.fperm.data();
+    m.steady = nullptr;
+    m.gamma = p->gamma;
+    m.eps = p->epsilon;
+    m.exponent = p->exponent;
+    for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
+    m.mode = p->reconstruction;
+    m.flux = p->flux;
+    m.eq = p->equilibrium;
+    m.gravity = p->gravity;
+    m.g = p->gravity_g;
+    for (int d = 0; d < 3; ++d) m.axis[d] = p->gravity_axis[d];
+    m.rhoC = p->polytrope_rhoC;
+    m.K = p->polytrope_K;
+    m.G = p->polytrope_G;
+    m.flux_bc = p->flux_bc;
+    std::memset(m.shift, 0, sizeof(m.shift));
+    const int fdim = im.s.dim == 2 ? 1 : 3;
+    for (size_t q = 0; q < im.face_w.size(); ++q) {
+      m.fw[q] = im.face_w[q];
+      for (int d = 0; d < fdim; ++d) m.fx[q][d] = im.face_x[q * fdim + d];
+    }
+    m.qv = int(im.vol_w.size());
+    for (size_t q = 0; q < im.vol_w.size(); ++q) {
+      m.vw[q] = im.vol_w[q];
+      for (int d = 0; d < im.s.dim + 1; ++d) m.vx[q][d] = im.vol_x[q * (im.s.dim + 1) + d];
+    }
+    const int nv = im.s.nvars;
+    e->fpv.assign(size_t(2) * im.s.q * nv * std::max<int64_t>(im.ne, 1), 0.0);
+    e->G.assign(size_t(nv) * std::max<int64_t>(im.ne, 1), 0.0);
+    if (p->gravity != 0) e->src.assign(size_t(nv) * im.n, 0.0);
+    *out = e.release();
+    return 0;
+  } catch (const std::exception &ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+int zfv_emu_destroy(void *h) {
+  delete static_cast<Emu *>(h);
+  return 0;
+}
+
+// dudt += L(u), both n x 5 row-major in caller order
+int zfv_emu_rate(void *h, const double *u, double *dudt) {
+  try {
+    Emu &e = *static_cast<Emu *>(h);
+    const HostImage &im = e.img;
+    const int64_t n = im.n;
+    const int nv = im.s.nvars, dim = im.s.dim;
+    std::vector<double> U(size_t(nv) * n), L(size_t(nv) * n);
+    for (int64_t ip = 0; ip < n; ++ip) {
+      const double *a = u + 5 * int64_t(im.perm[ip]);
+      U[ip] = a[0];
+      for (int d = 0; d < dim; ++d) U[(1 + d) * n + ip] = a[1 + d];
+      U[(nv - 1) * n + ip] = a[4];
+    }
+    dispatch(e, U.data(), L.data());
+    for (int64_t ip = 0; ip < n; ++ip) {
+      double *a = dudt + 5 * int64_t(im.perm[ip]);
+      a[0] += L[ip];
+      for (int d = 0; d < dim; ++d) a[1 + d] += L[(1 + d) * n + ip];
+      a[4] += L[(nv - 1) * n + ip];
+    }
+    return 0;
+  } catch (const std::exception &ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+int zfv_emu_stencils(void *h, int i, int *l2g, int cap, int *l2g_size, int *members, int members_cap) {
+  const HostImage &im = static_cast<Emu *>(h)->img;
+  const int a = im.l2g_off[i], b = im.l2g_off[i + 1];
+  *l2g_size = b - a;
+  for (int q = a; q < b && q - a < cap; ++q) l2g[q - a] = im.l2g[q];
+  const int ma = im.mem_off[i], mb = im.mem_off[i + 1];
+  if (mb - ma > members_cap) return -1;
+  for (int q = ma; q < mb; ++q) members[q - ma] = im.members[q];
+  return mb - ma;
+}
+
+int zfv_emu_info(void *h, int64_t *n, int64_t *ne, double *max_cond) {
+  const HostImage &im = static_cast<Emu *>(h)->img;
+  *n = im.n;
+  *ne = im.ne;
+  *max_cond = im.max_cond;
+  return 0;
+}
+
+}  // extern "C"

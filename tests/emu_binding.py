@@ -1,0 +1,69 @@
+"""ctypes binding of the test-only CPU emulation (tests/emu/libzfv_emu.so):
+the product's setup + kernel math run in host loops (see tests/emu/emu.cpp)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "emu", "build", "libzfv_emu.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    subprocess.run(["make", "-C", os.path.join(_HERE, "emu")], check=True, stdout=subprocess.DEVNULL)
+    from zisafvm_amd import CMesh, Params
+
+    lib = ctypes.CDLL(LIB)
+    vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.zfv_emu_create.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    lib.zfv_emu_destroy.argtypes = [vp]
+    lib.zfv_emu_rate.argtypes = [vp, dp, dp]
+    lib.zfv_emu_stencils.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int, ip, ip, ctypes.c_int]
+    lib.zfv_emu_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), dp]
+    lib.zfv_emu_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+class Emu:
+    def __init__(self, mesh, params):
+        self.lib = load()
+        self.mesh = mesh
+        self._cm = mesh.c_struct()
+        h = ctypes.c_void_p()
+        if self.lib.zfv_emu_create(ctypes.byref(self._cm), ctypes.byref(params), ctypes.byref(h)) != 0:
+            raise RuntimeError(self.lib.zfv_emu_last_error().decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.zfv_emu_destroy(self.h)
+
+    def rate(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u)
+        dp = ctypes.POINTER(ctypes.c_double)
+        if self.lib.zfv_emu_rate(self.h, u.ctypes.data_as(dp), out.ctypes.data_as(dp)) != 0:
+            raise RuntimeError(self.lib.zfv_emu_last_error().decode())
+        return out
+
+    def stencils(self, i):
+        l2g = (ctypes.c_int32 * 512)()
+        size = ctypes.c_int32()
+        mem = (ctypes.c_int32 * 1024)()
+        w = self.lib.zfv_emu_stencils(self.h, i, l2g, 512, ctypes.byref(size), mem, 1024)
+        assert w >= 0
+        return list(l2g[: size.value]), list(mem[:w])
+
+    def info(self):
+        n, ne, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        self.lib.zfv_emu_info(self.h, ctypes.byref(n), ctypes.byref(ne), ctypes.byref(c))
+        return n.value, ne.value, c.value

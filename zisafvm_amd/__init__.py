@@ -1,0 +1,365 @@
+"""zisafvm_amd — host-side (Python) access to libzfv_hip.so, the gfx950 FP64
+rate-of-change engine for unstructured CWENO-AO/WENO-AO finite volumes.
+
+The C-ABI is declared in ``include/zfv.h``; this module binds it with ctypes
+for tests and benchmarks.  The hot path is the HIP library: nothing here
+computes a rate of change, and a missing library raises instead of falling
+back to anything else.
+
+Reference interface mirrored (ZisaFVM): ``RateOfChange::compute``
+(include/zisa/ode/rate_of_change.hpp:37-39) -> :meth:`Solver.rate`;
+``TimeIntegration::compute_step`` (include/zisa/ode/time_integration.hpp:42-44)
+-> :meth:`Solver.step`; ``LocalCFL::operator()``
+(include/zisa/model/local_cfl_condition_impl.hpp:25-40) -> :meth:`Solver.cfl_dt`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libzfv_hip.so")
+
+ZFV_CWENO_AO, ZFV_WENO_AO = 0, 1
+ZFV_FLUX_HLLC, ZFV_FLUX_RUSANOV = 0, 1
+ZFV_EQ_NONE, ZFV_EQ_ISENTROPIC, ZFV_EQ_ISOTHERMAL = 0, 1, 2
+ZFV_GRAVITY_NONE, ZFV_GRAVITY_AXIAL, ZFV_GRAVITY_RADIAL, ZFV_GRAVITY_POLYTROPE = 0, 1, 2, 3
+
+
+class Params(ctypes.Structure):
+    """``zfv_params`` (include/zfv.h)."""
+
+    _fields_ = [
+        ("n_dims", ctypes.c_int32),
+        ("n_stencils", ctypes.c_int32),
+        ("orders", ctypes.c_int32 * 5),
+        ("biases", ctypes.c_int32 * 5),
+        ("overfit", ctypes.c_double * 5),
+        ("linear_weights", ctypes.c_double * 5),
+        ("epsilon", ctypes.c_double),
+        ("exponent", ctypes.c_double),
+        ("reconstruction", ctypes.c_int32),
+        ("flux", ctypes.c_int32),
+        ("equilibrium", ctypes.c_int32),
+        ("gravity", ctypes.c_int32),
+        ("gravity_g", ctypes.c_double),
+        ("gravity_axis", ctypes.c_double * 3),
+        ("polytrope_rhoC", ctypes.c_double),
+        ("polytrope_K", ctypes.c_double),
+        ("polytrope_G", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("gas_constant", ctypes.c_double),
+        ("edge_deg", ctypes.c_int32),
+        ("volume_deg", ctypes.c_int32),
+        ("moments_deg", ctypes.c_int32),
+        ("flux_bc", ctypes.c_int32),
+        ("steps_per_recompute", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("recompute_threshold", ctypes.c_double),
+    ]
+
+
+class CMesh(ctypes.Structure):
+    """``zfv_mesh`` (include/zfv.h)."""
+
+    _fields_ = [
+        ("n_dims", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32),
+        ("vertex_ids", ctypes.POINTER(ctypes.c_int32)),
+        ("corners", ctypes.POINTER(ctypes.c_double)),
+        ("ghost_mask", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int64),
+        ("n_faces", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("bytes_per_cell_stage", ctypes.c_double),
+        ("recon_bytes_per_cell", ctypes.c_double),
+        ("max_stencil_size", ctypes.c_int32),
+        ("n_points_face", ctypes.c_int32),
+    ]
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [
+        ("reconstruct_ms", ctypes.c_double),
+        ("face_flux_ms", ctypes.c_double),
+        ("update_ms", ctypes.c_double),
+        ("n_launches", ctypes.c_int32),
+    ]
+
+
+def make_params(
+    n_dims=2,
+    order=3,
+    n_biased=None,
+    overfit_central=2.0,
+    overfit_biased=1.5,
+    linear_weights=(100.0, 1.0, 1.0, 1.0, 1.0),
+    epsilon=1e-6,
+    exponent=4.0,
+    reconstruction=ZFV_CWENO_AO,
+    flux=ZFV_FLUX_HLLC,
+    equilibrium=ZFV_EQ_NONE,
+    gravity=ZFV_GRAVITY_NONE,
+    gravity_g=0.0,
+    gravity_axis=(0.0, 1.0, 0.0),
+    polytrope=(1.0, 1.0, 1.0),
+    gamma=1.4,
+    gas_constant=1.0,
+    edge_deg=None,
+    volume_deg=None,
+    moments_deg=-1,
+    flux_bc=0,
+) -> Params:
+    """Reconstruction parameters in the reference's shape
+    (src/zisa/reconstruction/hybrid_weno_params.cpp:26-44: orders {p,2,2,2},
+    biases {c,b,b,b}, overfit {2,1.5,1.5,1.5}, linear weights {100,1,1,1},
+    epsilon 1e-6, exponent 4)."""
+    p = Params()
+    nb = (n_dims if n_dims == 3 else 3) if n_biased is None else n_biased
+    p.n_dims = n_dims
+    p.n_stencils = 1 + nb
+    for k in range(5):
+        p.orders[k] = order if k == 0 else 2
+        p.biases[k] = 0 if k == 0 else 1
+        p.overfit[k] = overfit_central if k == 0 else overfit_biased
+        p.linear_weights[k] = linear_weights[k] if k < len(linear_weights) else 1.0
+    p.epsilon = epsilon
+    p.exponent = exponent
+    p.reconstruction = reconstruction
+    p.flux = flux
+    p.equilibrium = equilibrium
+    p.gravity = gravity
+    p.gravity_g = gravity_g
+    for d in range(3):
+        p.gravity_axis[d] = gravity_axis[d]
+    p.polytrope_rhoC, p.polytrope_K, p.polytrope_G = polytrope
+    p.gamma = gamma
+    p.gas_constant = gas_constant
+    if edge_deg is None:
+        edge_deg = order if n_dims == 2 else 2
+    if volume_deg is None:
+        volume_deg = 4 if n_dims == 2 else 3
+    p.edge_deg = edge_deg
+    p.volume_deg = volume_deg
+    p.moments_deg = moments_deg
+    p.flux_bc = flux_bc
+    p.steps_per_recompute = 1
+    p.recompute_threshold = 1e300
+    return p
+
+
+class ZfvError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libzfv_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ZfvError(f"{path} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    c_int, c_double, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    sigs = {
+        "zfv_create": (c_int, [ctypes.POINTER(CMesh), ctypes.POINTER(Params), ctypes.POINTER(vp)]),
+        "zfv_destroy": (c_int, [vp]),
+        "zfv_set_state": (c_int, [vp, dp]),
+        "zfv_get_state": (c_int, [vp, dp]),
+        "zfv_set_steady_state": (c_int, [vp, dp]),
+        "zfv_rate": (c_int, [vp, dp, dp, c_double]),
+        "zfv_step": (c_int, [vp, c_double, c_double]),
+        "zfv_steps": (c_int, [vp, ctypes.c_int32, c_double, c_double]),
+        "zfv_cfl_dt": (c_int, [vp, c_double, dp]),
+        "zfv_synchronize": (c_int, [vp]),
+        "zfv_get_info": (c_int, [vp, ctypes.POINTER(Info)]),
+        "zfv_get_stencils": (c_int, [vp, ctypes.c_int32, ip, ctypes.c_int32, ip, ip, ctypes.c_int32]),
+        "zfv_time_kernels": (c_int, [vp, ctypes.c_int32, c_double, ctypes.POINTER(KernelTimes)]),
+        "zfv_last_error": (ctypes.c_char_p, []),
+        "zfv_mesh_lattice2d": (
+            c_int,
+            [ctypes.c_int32, c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+             ctypes.POINTER(vp)],
+        ),
+        "zfv_mesh_ball3d": (
+            c_int, [ctypes.c_int32, c_double, c_double, c_double, ctypes.c_uint64, ctypes.POINTER(vp)]
+        ),
+        "zfv_mesh_view": (c_int, [vp, ctypes.POINTER(CMesh)]),
+        "zfv_mesh_free": (c_int, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().zfv_last_error()
+        raise ZfvError(f"zfv error {rc}: {msg.decode() if msg else ''}")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Mesh:
+    """A simplex mesh in the reference's layout plus per-cell corners."""
+
+    n_dims: int
+    vertex_ids: np.ndarray  # (n, d+1) int32
+    corners: np.ndarray  # (n, d+1, 3) float64
+    ghost: np.ndarray  # (n,) uint8
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.vertex_ids.shape[0])
+
+    def c_struct(self) -> CMesh:
+        self.vertex_ids = np.ascontiguousarray(self.vertex_ids, dtype=np.int32)
+        self.corners = np.ascontiguousarray(self.corners, dtype=np.float64)
+        self.ghost = np.ascontiguousarray(self.ghost, dtype=np.uint8)
+        m = CMesh()
+        m.n_dims = self.n_dims
+        m.n_cells = self.n_cells
+        m.vertex_ids = self.vertex_ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        m.corners = self.corners.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        m.ghost_mask = self.ghost.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        return m
+
+    def centroids(self) -> np.ndarray:
+        return self.corners.mean(axis=1)
+
+    def volumes(self) -> np.ndarray:
+        c = self.corners
+        if self.n_dims == 2:
+            a, b = c[:, 1] - c[:, 0], c[:, 2] - c[:, 0]
+            return 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
+        a, b, d = c[:, 1] - c[:, 0], c[:, 2] - c[:, 0], c[:, 3] - c[:, 0]
+        return np.abs(np.einsum("ij,ij->i", a, np.cross(b, d))) / 6.0
+
+
+def _mesh_from_buf(lib, buf) -> Mesh:
+    view = CMesh()
+    _check(lib.zfv_mesh_view(buf, ctypes.byref(view)))
+    n, d = view.n_cells, view.n_dims
+    vid = np.ctypeslib.as_array(view.vertex_ids, shape=(n * (d + 1),)).copy().reshape(n, d + 1)
+    crn = np.ctypeslib.as_array(view.corners, shape=(n * (d + 1) * 3,)).copy().reshape(n, d + 1, 3)
+    gh = np.ctypeslib.as_array(view.ghost_mask, shape=(n,)).copy()
+    lib.zfv_mesh_free(buf)
+    return Mesh(d, vid, crn, gh)
+
+
+def lattice_mesh_2d(n: int, jitter: float = 0.2, seed: int = 1, periodic=(True, True), ghost_layers: int = 3) -> Mesh:
+    """Seeded jittered n x n lattice on [0,1]^2, Delaunay diagonals (2 n^2 cells
+    per periodic domain; non-periodic directions get ghost layers)."""
+    lib = load_library()
+    buf = ctypes.c_void_p()
+    _check(lib.zfv_mesh_lattice2d(n, jitter, seed, int(periodic[0]), int(periodic[1]), ghost_layers, ctypes.byref(buf)))
+    return _mesh_from_buf(lib, buf)
+
+
+def ball_mesh_3d(n: int, radius: float = 1.0, ghost_radius: float = 0.85, jitter: float = 0.1, seed: int = 1) -> Mesh:
+    lib = load_library()
+    buf = ctypes.c_void_p()
+    _check(lib.zfv_mesh_ball3d(n, radius, ghost_radius, jitter, seed, ctypes.byref(buf)))
+    return _mesh_from_buf(lib, buf)
+
+
+class Solver:
+    """One device-resident instance (a ``zfv_handle``)."""
+
+    def __init__(self, mesh: Mesh, params: Params):
+        self.lib = load_library()
+        self.mesh = mesh
+        self.params = params
+        self._cm = mesh.c_struct()
+        h = ctypes.c_void_p()
+        _check(self.lib.zfv_create(ctypes.byref(self._cm), ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.zfv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_cells(self) -> int:
+        return self.mesh.n_cells
+
+    def set_state(self, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (self.n_cells, 5)
+        _check(self.lib.zfv_set_state(self.h, _dp(u)))
+
+    def get_state(self) -> np.ndarray:
+        u = np.zeros((self.n_cells, 5))
+        _check(self.lib.zfv_get_state(self.h, _dp(u)))
+        return u
+
+    def set_steady_state(self, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(self.lib.zfv_set_steady_state(self.h, _dp(u)))
+
+    def rate(self, u: np.ndarray, t: float = 0.0, dudt: Optional[np.ndarray] = None) -> np.ndarray:
+        """RateOfChange::compute: returns dudt + L(u) (dudt defaults to zeros)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u) if dudt is None else np.ascontiguousarray(dudt, dtype=np.float64)
+        _check(self.lib.zfv_rate(self.h, _dp(u), _dp(out), t))
+        return out
+
+    def step(self, t: float, dt: float):
+        _check(self.lib.zfv_step(self.h, t, dt))
+
+    def steps(self, n: int, t: float, dt: float):
+        _check(self.lib.zfv_steps(self.h, n, t, dt))
+
+    def cfl_dt(self, cfl: float) -> float:
+        dt = ctypes.c_double()
+        _check(self.lib.zfv_cfl_dt(self.h, cfl, ctypes.byref(dt)))
+        return dt.value
+
+    def synchronize(self):
+        _check(self.lib.zfv_synchronize(self.h))
+
+    def info(self) -> Info:
+        inf = Info()
+        _check(self.lib.zfv_get_info(self.h, ctypes.byref(inf)))
+        return inf
+
+    def stencils(self, i: int):
+        l2g = (ctypes.c_int32 * 512)()
+        size = ctypes.c_int32()
+        mem = (ctypes.c_int32 * 1024)()
+        w = self.lib.zfv_get_stencils(self.h, i, l2g, 512, ctypes.byref(size), mem, 1024)
+        if w < 0:
+            _check(w)
+        return list(l2g[: size.value]), list(mem[:w])
+
+    def time_kernels(self, n_steps: int, dt: float) -> KernelTimes:
+        kt = KernelTimes()
+        _check(self.lib.zfv_time_kernels(self.h, n_steps, dt, ctypes.byref(kt)))
+        return kt

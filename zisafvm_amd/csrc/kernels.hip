@@ -1,0 +1,622 @@
+// gfx950 kernels of the rate-of-change hot path and the C-ABI of
+// libzfv_hip.so (declared in include/zfv.h).
+//
+// One SSP3 stage = three launches over SoA arrays in HBM:
+//   k_reconstruct  one thread per cell: stencil gather, LSQ pseudo-inverse
+//                  mat-vecs, CWENO/WENO-AO, equilibrium solve (well-balanced),
+//                  point values on the cell's faces (face frame), gravity source
+//   k_face_flux    one thread per interior face: HLLC/Rusanov at the face
+//                  points, integrated once per face
+//   k_update       one thread per cell: atomic-free gather of the cell's face
+//                  fluxes, source, flux BC, Butcher-form stage sum, frozen BC
+// FP64 throughout; no MFMA (the per-cell mat-vecs are memory-bound).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "setup.hpp"
+#include "stage_math.hpp"
+
+namespace zfv {
+
+static thread_local std::string g_last_error;
+
+#define HIP_CHECK(expr)                                                                          \
+  do {                                                                                           \
+    hipError_t _e = (expr);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+      throw ::zfv::ZfvError(ZFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kernels
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+__global__ __launch_bounds__(256) void k_reconstruct(Img m, const double *__restrict__ U, double *__restrict__ fpv,
+                                                     double *__restrict__ src) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(m, i, U, fpv, src);
+}
+
+template <int DIM, int Q>
+__global__ __launch_bounds__(256) void k_face_flux(Img m, const double *__restrict__ fpv, double *__restrict__ G) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  face_flux<DIM, Q>(m, e, fpv, G);
+}
+
+struct StageCoeffs {
+  double c[4];  // coefficients of k_0 .. k_stage in the next state's sum
+  int nk;       // number of k's in the sum (stage + 1)
+  int write_rate_only;  // 1: store L in K[stage] and stop (zfv_rate)
+};
+
+// k_update: K[stage] = L(Ustage); Unext = U0 + dt * sum_j c_j K[j]
+// (src/zisa/ode/runge_kutta.cpp:122-143), then FrozenBC for ghost cells.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage, double dt,
+                                                const double *__restrict__ Ustage, const double *__restrict__ G,
+                                                const double *__restrict__ src, const double *U0,
+                                                double *__restrict__ K, double *Unext) {
+  constexpr int NV = DIM + 2;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = m.n;
+  if (i >= n) return;
+  double L[NV];
+  cell_rate<DIM>(m, i, Ustage, G, src, L);
+  for (int v = 0; v < NV; ++v) K[(int64_t(stage) * NV + v) * n + i] = L[v];
+  if (sc.write_rate_only) return;
+  const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
+  for (int v = 0; v < NV; ++v) {
+    double dudt = 0.0;
+    for (int j = 0; j < sc.nk; ++j) {
+      if (sc.c[j] != 0.0) {
+        const double kj = (j == stage) ? L[v] : K[(int64_t(j) * NV + v) * n + i];
+        dudt += sc.c[j] * kj;
+      }
+    }
+    double un = U0[v * n + i] + dt * dudt;
+    if (ghost) un = m.steady[v * n + i];
+    Unext[v * n + i] = un;
+  }
+}
+
+// LocalCFL: min_i inradius_i / (|v_i| + a_i); positive doubles order like
+// their bit patterns, so an unsigned 64-bit atomic min reduces them.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U, unsigned long long *out) {
+  constexpr int NV = DIM + 2;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double r = 1.0e300;
+  if (i < m.n) {
+    const int64_t n = m.n;
+    double u[NV];
+    for (int v = 0; v < NV; ++v) u[v] = U[v * n + i];
+    double m2 = u[1] * u[1] + u[2] * u[2];
+    if (DIM == 3) m2 += u[3] * u[3];
+    double v2 = m2 / (u[0] * u[0]);
+    double p = (u[NV - 1] - 0.5 * m2 / u[0]) * (m.gamma - 1.0);
+    double a = sqrt(m.gamma * p / u[0]);
+    r = m.inr[i] / (sqrt(v2) + a);
+    if (!(r > 0.0)) r = 0.0;  // NaN / non-positive states surface as dt = 0
+  }
+  // wavefront reduction
+  for (int off = 32; off > 0; off >>= 1) r = fmin(r, __shfl_down(r, off, 64));
+  __shared__ double wmin[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wmin[wid] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wmin[0];
+    for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmin(b, wmin[w]);
+    unsigned long long bits;
+    memcpy(&bits, &b, sizeof(bits));
+    atomicMin(out, bits);
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// launch plan for one shape
+
+struct Launcher {
+  void (*rec)(const Img &, const double *, double *, double *, hipStream_t);
+  void (*face)(const Img &, const double *, double *, hipStream_t);
+  void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
+                 const double *, double *, double *, hipStream_t);
+  void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
+};
+
+inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+Launcher make_launcher() {
+  Launcher l;
+  l.rec = [](const Img &m, const double *U, double *fpv, double *src, hipStream_t s) {
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U,
+                       fpv, src);
+  };
+  l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
+    if (m.ne > 0)
+      hipLaunchKernelGGL((k_face_flux<DIM, Q>), dim3(blocks_for(m.ne, 256)), dim3(256), 0, s, m, fpv, G);
+  };
+  l.update = [](const Img &m, StageCoeffs sc, int stage, double dt, const double *Us, const double *G,
+                const double *src, const double *U0, double *K, double *Un, hipStream_t s) {
+    hipLaunchKernelGGL((k_update<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, sc, stage, dt, Us, G, src,
+                       U0, K, Un);
+  };
+  l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
+  };
+  return l;
+}
+
+Launcher pick_launcher(const Shape &s) {
+  // instantiated shapes: (dim, dof_central, central rows, one-sided stencils, rows each, face points)
+#define ZFV_SHAPE(D, DOF, RC_, NB_, RB_, Q_) \
+  if (s.dim == D && s.dofc == DOF && s.rc == RC_ && s.nb == NB_ && s.rb == RB_ && s.q == Q_) \
+    return make_launcher<D, DOF, RC_, NB_, RB_, Q_>();
+  ZFV_SHAPE(2, 6, 10, 3, 3, 2)   // 2D CWENO-AO order 3 {3,2,2,2}, overfit {2,1.5,...}, edge deg 2-3
+  ZFV_SHAPE(2, 6, 10, 3, 3, 3)   // ... edge deg 4-5
+  ZFV_SHAPE(2, 10, 18, 3, 3, 3)  // 2D order 4, edge deg 4-5
+  ZFV_SHAPE(2, 10, 18, 3, 3, 2)  // 2D order 4, edge deg 2-3
+  ZFV_SHAPE(2, 3, 4, 3, 3, 1)    // 2D order 2 (overfit 2.0 central), midpoint rule
+  ZFV_SHAPE(2, 3, 4, 3, 3, 2)
+  ZFV_SHAPE(3, 10, 18, 4, 4, 3)  // 3D order 3 {3,2,2,2,2}, face deg 2
+  ZFV_SHAPE(3, 10, 18, 4, 4, 4)  // ... face deg 3
+  ZFV_SHAPE(3, 10, 18, 3, 4, 3)  // 3D order 3 with the reference default {3,2,2,2}
+#undef ZFV_SHAPE
+  throw ZfvError(ZFV_ERR_INVALID, "no kernel instantiation for this (dimension, order, stencil, quadrature) shape");
+}
+
+}  // namespace zfv
+
+// ---------------------------------------------------------------------------
+// the handle
+
+struct zfv_handle {
+  zfv::HostImage img;
+  zfv::Img dimg{};
+  zfv::Launcher L{};
+  zfv_params params{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocations;
+  int64_t device_bytes = 0;
+  // state buffers (device)
+  double *U = nullptr, *X = nullptr, *K = nullptr, *fpv = nullptr, *G = nullptr, *src = nullptr;
+  double *steady = nullptr;
+  unsigned long long *dt_bits = nullptr;
+  double *host_stage = nullptr;  // pinned staging n * nvars
+  bool has_steady = false;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = -1.0;
+  int graph_steps = 0;
+
+  template <class T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIP_CHECK(hipMalloc(&p, bytes));
+    allocations.push_back(p);
+    device_bytes += int64_t(bytes);
+    return static_cast<T *>(p);
+  }
+  template <class T>
+  const T *upload(const std::vector<T> &v) {
+    T *p = alloc<T>(v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  ~zfv_handle() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    for (void *p : allocations) (void)hipFree(p);
+    if (host_stage) (void)hipHostFree(host_stage);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace zfv {
+
+static int fail(const std::exception &e) {
+  g_last_error = e.what();
+  if (auto *z = dynamic_cast<const ZfvError *>(&e)) return z->code;
+  return ZFV_ERR_INVALID;
+}
+
+#define ZFV_API_BEGIN try {
+#define ZFV_API_END                     \
+  }                                     \
+  catch (const std::exception &e) {     \
+    return fail(e);                     \
+  }
+
+static void check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw ZfvError(ZFV_ERR_NO_DEVICE, "no HIP device visible");
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    throw ZfvError(ZFV_ERR_NO_DEVICE, std::string("libzfv_hip is built for gfx950; device is ") + prop.gcnArchName);
+}
+
+static StageCoeffs ssp3_coeffs(int stage) {
+  // next-state coefficients after stage s: a[s+1][:] and b for the last stage
+  // (src/zisa/ode/runge_kutta.cpp:154-165)
+  StageCoeffs c{};
+  if (stage == 0) {
+    c.c[0] = 1.0;
+    c.nk = 1;
+  } else if (stage == 1) {
+    c.c[0] = 0.25;
+    c.c[1] = 0.25;
+    c.nk = 2;
+  } else {
+    c.c[0] = 1.0 / 6;
+    c.c[1] = 1.0 / 6;
+    c.c[2] = 2.0 / 3;
+    c.nk = 3;
+  }
+  c.write_rate_only = 0;
+  return c;
+}
+
+static void enqueue_step(zfv_handle *h, double dt) {
+  const Img &m = h->dimg;
+  const double *Us = h->U;
+  for (int s = 0; s < 3; ++s) {
+    h->L.rec(m, Us, h->fpv, h->src, h->stream);
+    h->L.face(m, h->fpv, h->G, h->stream);
+    double *Un = (s == 2) ? h->U : h->X;
+    h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+    Us = h->X;
+  }
+}
+
+static void to_soa(const zfv_handle *h, const double *aos, double *soa) {
+  const int64_t n = h->img.n;
+  const int nv = h->img.s.nvars;
+  const int dim = h->img.s.dim;
+  for (int64_t ip = 0; ip < n; ++ip) {
+    const double *u = aos + 5 * int64_t(h->img.perm[ip]);
+    soa[0 * n + ip] = u[0];
+    for (int d = 0; d < dim; ++d) soa[(1 + d) * n + ip] = u[1 + d];
+    soa[(nv - 1) * n + ip] = u[4];
+  }
+}
+
+static void from_soa(const zfv_handle *h, const double *soa, double *aos, bool accumulate) {
+  const int64_t n = h->img.n;
+  const int nv = h->img.s.nvars;
+  const int dim = h->img.s.dim;
+  for (int64_t ip = 0; ip < n; ++ip) {
+    double *u = aos + 5 * int64_t(h->img.perm[ip]);
+    double w[5] = {soa[0 * n + ip], 0.0, 0.0, 0.0, soa[(nv - 1) * n + ip]};
+    for (int d = 0; d < dim; ++d) w[1 + d] = soa[(1 + d) * n + ip];
+    for (int v = 0; v < 5; ++v) u[v] = accumulate ? u[v] + w[v] : w[v];
+  }
+}
+
+}  // namespace zfv
+
+using namespace zfv;
+
+extern "C" {
+
+const char *zfv_last_error(void) { return g_last_error.c_str(); }
+
+int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out) {
+  ZFV_API_BEGIN
+  if (!mesh || !params || !out) throw ZfvError(ZFV_ERR_INVALID, "null argument");
+  check_device();
+  auto h = std::make_unique<zfv_handle>();
+  h->params = *params;
+  build_image(*mesh, *params, h->img);
+  const HostImage &im = h->img;
+  const Shape &s = im.s;
+  h->L = pick_launcher(s);
+  HIP_CHECK(hipGetDevice(&h->dev));
+  HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+  Img &m = h->dimg;
+  m.n = im.n;
+  m.ne = im.ne;
+  m.P = h->upload(im.P);
+  m.rows = h->upload(im.rows);
+  m.crn = h->upload(im.crn);
+  m.xc = h->upload(im.xc);
+  m.len = h->upload(im.len);
+  m.vol = h->upload(im.vol);
+  m.inr = h->upload(im.inr);
+  m.mom = h->upload(im.mom);
+  m.meta = h->upload(im.meta);
+  m.cface = h->upload(im.cface);
+  m.fperm = h->upload(im.fperm);
+  m.steady = nullptr;
+  m.gamma = params->gamma;
+  m.eps = params->epsilon;
+  m.exponent = params->exponent;
+  for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
+  m.mode = params->reconstruction;
+  m.flux = params->flux;
+  m.eq = params->equilibrium;
+  m.gravity = params->gravity;
+  m.g = params->gravity_g;
+  for (int d = 0; d < 3; ++d) m.axis[d] = params->gravity_axis[d];
+  m.rhoC = params->polytrope_rhoC;
+  m.K = params->polytrope_K;
+  m.G = params->polytrope_G;
+  m.flux_bc = params->flux_bc;
+  std::memset(m.shift, 0, sizeof(m.shift));
+  if (im.face_w.size() > 8 || im.vol_w.size() > 16) throw ZfvError(ZFV_ERR_INVALID, "quadrature rule too large");
+  const int fdim = s.dim == 2 ? 1 : 3;
+  for (size_t q = 0; q < im.face_w.size(); ++q) {
+    m.fw[q] = im.face_w[q];
+    for (int d = 0; d < fdim; ++d) m.fx[q][d] = im.face_x[q * fdim + d];
+  }
+  m.qv = int(im.vol_w.size());
+  for (size_t q = 0; q < im.vol_w.size(); ++q) {
+    m.vw[q] = im.vol_w[q];
+    for (int d = 0; d < s.dim + 1; ++d) m.vx[q][d] = im.vol_x[q * (s.dim + 1) + d];
+  }
+
+  const int64_t n = im.n;
+  const int nv = s.nvars;
+  h->U = h->alloc<double>(size_t(nv) * n);
+  h->X = h->alloc<double>(size_t(nv) * n);
+  h->K = h->alloc<double>(size_t(3) * nv * n);
+  h->fpv = h->alloc<double>(size_t(2) * s.q * nv * std::max<int64_t>(im.ne, 1));
+  h->G = h->alloc<double>(size_t(nv) * std::max<int64_t>(im.ne, 1));
+  if (params->gravity != ZFV_GRAVITY_NONE) h->src = h->alloc<double>(size_t(nv) * n);
+  h->dt_bits = h->alloc<unsigned long long>(1);
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
+  *out = h.release();
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_destroy(zfv_handle *h) {
+  delete h;
+  return ZFV_OK;
+}
+
+int zfv_set_state(zfv_handle *h, const double *cvars) {
+  ZFV_API_BEGIN
+  to_soa(h, cvars, h->host_stage);
+  const size_t bytes = sizeof(double) * h->img.s.nvars * h->img.n;
+  HIP_CHECK(hipMemcpyAsync(h->U, h->host_stage, bytes, hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_get_state(zfv_handle *h, double *cvars) {
+  ZFV_API_BEGIN
+  const size_t bytes = sizeof(double) * h->img.s.nvars * h->img.n;
+  HIP_CHECK(hipMemcpyAsync(h->host_stage, h->U, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  from_soa(h, h->host_stage, cvars, false);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_set_steady_state(zfv_handle *h, const double *cvars) {
+  ZFV_API_BEGIN
+  const int64_t n = h->img.n;
+  const int nv = h->img.s.nvars;
+  if (!h->steady) h->steady = h->alloc<double>(size_t(nv) * n);
+  to_soa(h, cvars, h->host_stage);
+  HIP_CHECK(hipMemcpyAsync(h->steady, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  h->dimg.steady = h->steady;
+  if (h->graph_exec) {
+    (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+  }
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
+  (void)t;
+  ZFV_API_BEGIN
+  const int64_t n = h->img.n;
+  const int nv = h->img.s.nvars;
+  to_soa(h, u, h->host_stage);
+  HIP_CHECK(hipMemcpyAsync(h->X, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
+  const Img &m = h->dimg;
+  h->L.rec(m, h->X, h->fpv, h->src, h->stream);
+  h->L.face(m, h->fpv, h->G, h->stream);
+  StageCoeffs sc{};
+  sc.write_rate_only = 1;
+  h->L.update(m, sc, 0, 0.0, h->X, h->G, h->src, h->X, h->K, h->X, h->stream);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(h->host_stage, h->K, sizeof(double) * nv * n, hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  from_soa(h, h->host_stage, dudt, true);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_step(zfv_handle *h, double t, double dt) {
+  (void)t;
+  ZFV_API_BEGIN
+  enqueue_step(h, dt);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
+  (void)t;
+  ZFV_API_BEGIN
+  if (n_steps < 0) throw ZfvError(ZFV_ERR_INVALID, "n_steps < 0");
+  if (!h->graph_exec || h->graph_dt != dt) {
+    if (h->graph_exec) {
+      HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
+      h->graph_exec = nullptr;
+    }
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h, dt);
+    HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
+    HIP_CHECK(hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    h->graph_dt = dt;
+  }
+  for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(h->graph_exec, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_cfl_dt(zfv_handle *h, double cfl_number, double *dt) {
+  ZFV_API_BEGIN
+  const double big = 1.0e300;
+  unsigned long long bits;
+  std::memcpy(&bits, &big, sizeof(bits));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, h->stream, h->dt_bits, bits);
+  h->L.cfl(h->dimg, h->U, h->dt_bits, h->stream);
+  HIP_CHECK(hipGetLastError());
+  unsigned long long r = 0;
+  HIP_CHECK(hipMemcpyAsync(&r, h->dt_bits, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  double m;
+  std::memcpy(&m, &r, sizeof(m));
+  *dt = cfl_number * m;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_synchronize(zfv_handle *h) {
+  ZFV_API_BEGIN
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_get_info(zfv_handle *h, zfv_info *info) {
+  ZFV_API_BEGIN
+  const Shape &s = h->img.s;
+  const double n = double(h->img.n), ne = double(h->img.ne);
+  info->n_cells = h->img.n;
+  info->n_faces = h->img.ne;
+  info->device_bytes = h->device_bytes;
+  // algorithmic bytes of the reconstruction kernel per cell: pseudo-inverse,
+  // stencil indices, own state, geometry (corners, centre, length, volume,
+  // moments), meta/face words, and the face-point values written once per
+  // interior face side
+  const double nv = s.nvars;
+  const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
+                     4.0 * (1 + s.nfaces + 1) + 8.0 * (2.0 * ne / n) * s.q * nv;
+  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv);
+  const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * s.dim + 1) + 4.0 + 8.0 * nv * 5;
+  info->recon_bytes_per_cell = rec;
+  info->bytes_per_cell_stage = rec + face + upd;
+  int maxs = 0;
+  for (size_t i = 0; i + 1 < h->img.l2g_off.size(); ++i)
+    maxs = std::max(maxs, h->img.l2g_off[i + 1] - h->img.l2g_off[i]);
+  info->max_stencil_size = maxs;
+  info->n_points_face = s.q;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_t *l2g_size, int32_t *members,
+                     int32_t members_cap) {
+  ZFV_API_BEGIN
+  const HostImage &im = h->img;
+  if (i < 0 || i >= im.n) throw ZfvError(ZFV_ERR_INVALID, "cell index out of range");
+  const int a = im.l2g_off[i], b = im.l2g_off[i + 1];
+  *l2g_size = b - a;
+  for (int q = a; q < b && q - a < cap; ++q) l2g[q - a] = im.l2g[q];
+  const int ma = im.mem_off[i], mb = im.mem_off[i + 1];
+  if (mb - ma > members_cap) throw ZfvError(ZFV_ERR_INVALID, "members buffer too small");
+  for (int q = ma; q < mb; ++q) members[q - ma] = im.members[q];
+  return mb - ma;
+  ZFV_API_END
+}
+
+int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out) {
+  ZFV_API_BEGIN
+  const Img &m = h->dimg;
+  std::vector<hipEvent_t> ev(2 * 9 * std::max(n_steps, 1));
+  for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+  int k = 0;
+  for (int st = 0; st < n_steps; ++st) {
+    const double *Us = h->U;
+    for (int s = 0; s < 3; ++s) {
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      h->L.rec(m, Us, h->fpv, h->src, h->stream);
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      h->L.face(m, h->fpv, h->G, h->stream);
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      double *Un = (s == 2) ? h->U : h->X;
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+      HIP_CHECK(hipEventRecord(ev[k++], h->stream));
+      Us = h->X;
+    }
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  zfv_kernel_times t{};
+  for (int q = 0; q < k; q += 6) {
+    float a = 0, b = 0, c = 0;
+    HIP_CHECK(hipEventElapsedTime(&a, ev[q], ev[q + 1]));
+    HIP_CHECK(hipEventElapsedTime(&b, ev[q + 2], ev[q + 3]));
+    HIP_CHECK(hipEventElapsedTime(&c, ev[q + 4], ev[q + 5]));
+    t.reconstruct_ms += a;
+    t.face_flux_ms += b;
+    t.update_ms += c;
+    t.n_launches += 1;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *out = t;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_mesh_lattice2d(int32_t n, double jitter, uint64_t seed, int32_t periodic_x, int32_t periodic_y,
+                       int32_t ghost_layers, zfv_mesh_buf **out) {
+  ZFV_API_BEGIN
+  *out = reinterpret_cast<zfv_mesh_buf *>(mesh_lattice2d(n, jitter, seed, periodic_x != 0, periodic_y != 0, ghost_layers));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_mesh_ball3d(int32_t n, double radius, double ghost_radius, double jitter, uint64_t seed, zfv_mesh_buf **out) {
+  ZFV_API_BEGIN
+  *out = reinterpret_cast<zfv_mesh_buf *>(mesh_ball3d(n, radius, ghost_radius, jitter, seed));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_mesh_view(const zfv_mesh_buf *buf, zfv_mesh *view) {
+  const MeshBuf *m = reinterpret_cast<const MeshBuf *>(buf);
+  view->n_dims = m->n_dims;
+  view->n_cells = m->n_cells;
+  view->vertex_ids = m->vertex_ids.data();
+  view->corners = m->corners.data();
+  view->ghost_mask = m->ghost.data();
+  return ZFV_OK;
+}
+
+int zfv_mesh_free(zfv_mesh_buf *buf) {
+  delete reinterpret_cast<MeshBuf *>(buf);
+  return ZFV_OK;
+}
+
+}  // extern "C"

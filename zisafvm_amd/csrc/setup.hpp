@@ -1,0 +1,74 @@
+// Host setup of one instance: geometry, stencils, least-squares
+// pseudo-inverses and the structure-of-arrays image the kernels read.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace zfv {
+
+// Compile-time shape family of the kernels (see kernels.hip for the
+// instantiations that exist).
+struct Shape {
+  int dim = 2;      // 2 or 3
+  int nvars = 4;    // conserved variables stored: rho, m (dim), E
+  int nfaces = 3;   // faces per cell
+  int q = 2;        // quadrature points per face
+  int qv = 6;       // quadrature points per cell (volume rule)
+  int dofc = 6;     // dof of the highest-order (central) stencil's polynomial
+  int rc = 10;      // rows of the central LSQ system (members - 1)
+  int nb = 3;       // number of one-sided stencils
+  int dofb = 3;     // dof of the one-sided polynomials
+  int rb = 3;       // rows of each one-sided LSQ system
+  int m = 68;       // pseudo-inverse entries per cell
+  int nrows = 19;   // stencil rows per cell = rc + nb*rb
+  int nmom = 3;     // stored moments (indices 3 .. dofc-1)
+};
+
+// Per-cell meta bits
+enum : int32_t {
+  META_KHIGH_MASK = 0x7,      // index of the stencil that receives the CWENO correction
+  META_FIRST_ORDER = 1 << 3,  // single one-cell stencil: constant reconstruction
+  META_GHOST = 1 << 4,        // frozen ghost cell
+};
+
+// cface bits: (face_id << 2) | (side << 0) | (exterior << 1)
+constexpr int CFACE_SIDE_R = 1;
+constexpr int CFACE_EXTERIOR = 2;
+
+struct HostImage {
+  Shape s;
+  int64_t n = 0;   // cells
+  int64_t ne = 0;  // interior faces
+  std::vector<double> P;         // m * n
+  std::vector<int32_t> rows;     // nrows * n   (product numbering)
+  std::vector<double> crn;       // (nfaces*dim) * n, corner k coordinate d at (k*dim+d)
+  std::vector<double> xc;        // dim * n
+  std::vector<double> len, vol, inr;
+  std::vector<double> mom;       // nmom * n
+  std::vector<int32_t> meta;     // n
+  std::vector<int32_t> cface;    // nfaces * n
+  std::vector<uint32_t> fperm;   // n: own-corner index of the face's left-cell vertex order
+  std::vector<int32_t> perm;     // product index -> caller index
+  std::vector<int32_t> iperm;    // caller index -> product index
+  std::vector<uint8_t> ghost;    // product numbering
+  // reference-layout bookkeeping for zfv_get_stencils (caller numbering)
+  std::vector<int32_t> l2g_off, l2g;               // CSR of the combined stencil
+  std::vector<int32_t> mem_off, members;           // CSR of [order, size, ids...]
+  // normalized quadrature rules used by the kernels (device constants)
+  std::vector<double> face_w, face_x;  // q weights; face_x: q * (dim) barycentric / GL coordinates
+  std::vector<double> vol_w, vol_x;    // qv weights; vol_x: qv * (dim+1) barycentric
+  double max_cond = 0.0;               // largest estimated condition number of A^T A
+  double linear_weights[ZFV_MAX_STENCILS] = {0};
+};
+
+// Builds everything on the host.  Throws ZfvError on unsupported input.
+void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img);
+
+// Shape implied by the parameters (throws if the GPU kernels have no
+// instantiation for it).
+Shape shape_for(const zfv_params &p);
+
+}  // namespace zfv

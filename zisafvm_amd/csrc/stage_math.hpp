@@ -1,0 +1,868 @@
+// Per-cell and per-face work of one rate-of-change evaluation, written once
+// for the gfx950 kernels (kernels.hip).  Everything is FP64.
+//
+// Reference semantics restated here:
+//   reconstruction   include/zisa/reconstruction/local_reconstruction.hpp:69-156,
+//                    src/zisa/reconstruction/{cweno_ao,weno_ao,hybrid_weno}.cpp,
+//                    include/zisa/math/poly2d_impl.hpp:271-337
+//   face flux        include/zisa/fvm_loops/flux_loop.hpp:126-193,
+//                    include/zisa/flux/hllc.hpp:17-176
+//   gravity source   include/zisa/fvm_loops/gravity_source_loop.hpp:32-148
+//   equilibria       include/zisa/model/local_equilibrium_impl.hpp:41-93,
+//                    include/zisa/model/isentropic_equilibrium.hpp:28-63
+//   flux BC          include/zisa/boundary/flux_bc.hpp:25-42
+//   RK stage         src/zisa/ode/runge_kutta.cpp:87-143
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ZFV_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define ZFV_HD inline
+#endif
+
+namespace zfv {
+
+constexpr int ROW_CELL_BITS = 27;
+constexpr int32_t ROW_CELL_MASK = (1 << ROW_CELL_BITS) - 1;
+
+// What the kernels read.  Pointers are device pointers (or host pointers in
+// the CPU emulation used by the tests).
+struct Img {
+  int64_t n;   // cells
+  int64_t ne;  // interior faces
+  const double *P;
+  const int32_t *rows;  // cell | shift_code << 27
+  const double *crn;
+  const double *xc;
+  const double *len;
+  const double *vol;
+  const double *inr;
+  const double *mom;
+  const int32_t *meta;
+  const int32_t *cface;
+  const uint32_t *fperm;
+  const double *steady;  // nvars * n (ghost cells only are read)
+  // physics
+  double gamma;
+  double eps, exponent;
+  double lw[5];  // normalized linear weights
+  int mode;      // 0 CWENO-AO, 1 WENO-AO
+  int flux;      // 0 HLLC, 1 Rusanov
+  int eq;        // 0 none, 1 isentropic, 2 isothermal
+  int gravity;   // 0 none, 1 axial, 2 radial, 3 polytrope
+  double g, axis[3], rhoC, K, G;
+  int flux_bc;
+  double shift[16][3];  // periodic translations addressed by the row shift code
+  // normalized quadrature rules
+  int qv;                   // points of the volume rule
+  double fw[8], fx[8][3];   // face rule: weights, GL abscissa (2D) or barycentric (3D)
+  double vw[16], vx[16][4]; // volume rule
+};
+
+template <int DIM>
+struct Dims {
+  static constexpr int NV = DIM + 2;
+  static constexpr int NF = DIM + 1;
+  static constexpr int DOFB = DIM + 1;
+};
+
+ZFV_HD double sqr(double x) { return x * x; }
+
+ZFV_HD double dmin(double a, double b) { return a < b ? a : b; }
+ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
+
+// ---- gravity -----------------------------------------------------------------
+template <int DIM>
+ZFV_HD double phi(const Img &m, const double *x) {
+  if (m.gravity == 1) {
+    double d = x[0] * m.axis[0] + x[1] * m.axis[1];
+    if (DIM == 3) d += x[2] * m.axis[2];
+    return m.g * d;
+  }
+  double r2 = x[0] * x[0] + x[1] * x[1];
+  if (DIM == 3) r2 += x[2] * x[2];
+  double r = sqrt(r2);
+  if (m.gravity == 2) return m.g * r;
+  if (m.gravity == 3) {
+    double alpha = sqrt(2.0 * 3.14159265358979323846 * m.G / m.K);
+    double ce = alpha * (r + 2.2250738585072014e-308);
+    return -2.0 * m.K * m.rhoC * sin(ce) / ce;
+  }
+  return 0.0;
+}
+
+template <int DIM>
+ZFV_HD void grad_phi(const Img &m, const double *x, double *gp) {
+  if (m.gravity == 1) {
+    for (int d = 0; d < DIM; ++d) gp[d] = m.g * m.axis[d];
+    return;
+  }
+  double r2 = x[0] * x[0] + x[1] * x[1];
+  if (DIM == 3) r2 += x[2] * x[2];
+  double r = sqrt(r2);
+  double dd = 0.0;
+  if (m.gravity == 2) {
+    dd = m.g;
+  } else if (m.gravity == 3) {
+    double alpha = sqrt(2.0 * 3.14159265358979323846 * m.G / m.K);
+    double ce = alpha * (r + 2.2250738585072014e-308);
+    double dphi = (cos(ce) - sin(ce) / ce) / ce;
+    dd = -2.0 * m.K * m.rhoC * dphi * alpha;
+  }
+  for (int d = 0; d < DIM; ++d) gp[d] = dd * (x[d] / (r + 1e-50));
+}
+
+// ---- local equilibrium: theta -> (rho, E) at x ------------------------------
+template <int DIM>
+ZFV_HD void eq_eval(const Img &m, const double *th, double phi_ref, const double *x, double &rho, double &E) {
+  const double gamma = m.gamma;
+  if (m.eq == 1) {
+    double h = th[0] + phi_ref - phi<DIM>(m, x);
+    double K = th[1];
+    double base = 1.0 / K * (gamma - 1.0) / gamma * h;
+    rho = pow(base, 1.0 / (gamma - 1.0));
+    double p = K * pow(rho, gamma);
+    E = p / (gamma - 1.0);
+  } else {
+    double RT = th[1];
+    rho = th[0] * exp(-(phi<DIM>(m, x) - phi_ref) / RT);
+    E = rho * RT / (gamma - 1.0);
+  }
+}
+
+// ---- geometry helpers ---------------------------------------------------------
+template <int DIM>
+ZFV_HD void load_corners(const Img &m, int64_t i, double (*v)[3]) {
+  for (int k = 0; k < DIM + 1; ++k) {
+    for (int d = 0; d < DIM; ++d) v[k][d] = m.crn[(k * DIM + d) * m.n + i];
+    if (DIM == 2) v[k][2] = 0.0;
+  }
+}
+
+// vertices of local face f in the order of the face's left cell
+template <int DIM>
+ZFV_HD void face_vertices(uint32_t fperm, int f, const double (*v)[3], double (*w)[3]) {
+  for (int r = 0; r < DIM; ++r) {
+    int c = (fperm >> (2 * (f * 3 + r))) & 3u;
+    w[r][0] = v[c][0];
+    w[r][1] = v[c][1];
+    w[r][2] = v[c][2];
+  }
+}
+
+// Face frame (src/zisa/math/face_factory.cpp:11-34): n, t1, t2 and the face
+// measure (edge length / Heron area).
+template <int DIM>
+ZFV_HD void face_frame(const double (*w)[3], double *n, double *t1, double *t2, double &area) {
+  if (DIM == 2) {
+    double dx = w[1][0] - w[0][0], dy = w[1][1] - w[0][1];
+    double l = sqrt(dx * dx + dy * dy);
+    area = l;
+    t1[0] = dx / l;
+    t1[1] = dy / l;
+    t1[2] = 0.0 / l;
+    n[0] = t1[1];
+    n[1] = -t1[0];
+    n[2] = t1[2];
+  } else {
+    double a[3] = {w[1][0] - w[0][0], w[1][1] - w[0][1], w[1][2] - w[0][2]};
+    double b[3] = {w[2][0] - w[0][0], w[2][1] - w[0][1], w[2][2] - w[0][2]};
+    double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    n[0] = c[0] / lc;
+    n[1] = c[1] / lc;
+    n[2] = c[2] / lc;
+    double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    t1[0] = a[0] / la;
+    t1[1] = a[1] / la;
+    t1[2] = a[2] / la;
+    // Heron on the side lengths (src/zisa/math/triangle.cpp:11-18)
+    double e0[3] = {w[1][0] - w[2][0], w[1][1] - w[2][1], w[1][2] - w[2][2]};
+    double e1[3] = {w[0][0] - w[2][0], w[0][1] - w[2][1], w[0][2] - w[2][2]};
+    double sa = sqrt(e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2]);
+    double sb = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    double sc = la;
+    double s = 0.5 * (sa + sb + sc);
+    area = sqrt(s * (s - sa) * (s - sb) * (s - sc));
+  }
+  t2[0] = n[1] * t1[2] - n[2] * t1[1];
+  t2[1] = n[2] * t1[0] - n[0] * t1[2];
+  t2[2] = n[0] * t1[1] - n[1] * t1[0];
+}
+
+// quadrature point q of a face given its vertices in left-cell order
+template <int DIM>
+ZFV_HD void face_point(const Img &m, int q, const double (*w)[3], double *x) {
+  if (DIM == 2) {
+    double xi = m.fx[q][0];
+    double a = 0.5 - 0.5 * xi, b = 0.5 + 0.5 * xi;
+    x[0] = a * w[0][0] + b * w[1][0];
+    x[1] = a * w[0][1] + b * w[1][1];
+    x[2] = 0.0;
+  } else {
+    for (int d = 0; d < 3; ++d) x[d] = (m.fx[q][0] * w[0][d] + m.fx[q][1] * w[1][d]) + m.fx[q][2] * w[2][d];
+  }
+}
+
+template <int DIM>
+ZFV_HD void volume_point(const Img &m, int q, const double (*v)[3], double *x) {
+  for (int d = 0; d < 3; ++d) {
+    double s = m.vx[q][0] * v[0][d];
+    for (int k = 1; k < DIM + 1; ++k) s = s + m.vx[q][k] * v[k][d];
+    x[d] = s;
+  }
+}
+
+// ---- polynomial evaluation (include/zisa/math/poly2d_impl.hpp:271-323) -------
+// a[alpha][v] for alpha < DOF, c = moments (0 for alpha < 3)
+template <int DIM, int DEG, int NV>
+ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, double *px) {
+  for (int v = 0; v < NV; ++v) px[v] = 0.0;
+  double pxk = 1.0;
+#pragma unroll
+  for (int kx = 0; kx <= DEG; ++kx) {
+    double pyk = 1.0;
+#pragma unroll
+    for (int ky = 0; ky <= DEG - kx; ++ky) {
+      if (DIM == 2) {
+        const int n = kx + ky;
+        const int i = (n + 1) * n / 2 + ky;
+        const double b = pxk * pyk - c[i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) px[v] += a[i][v] * b;
+      } else {
+        double pzk = 1.0;
+#pragma unroll
+        for (int kz = 0; kz <= DEG - kx - ky; ++kz) {
+          const int n3 = kx + ky + kz - 1;
+          const int d3 = n3 < 0 ? 0 : (n3 + 1) * (n3 + 2) * (n3 + 3) / 6;
+          const int n2 = ky + kz;
+          const int i = d3 + (n2 + 1) * n2 / 2 + kz;
+          const double b = pxk * pyk * pzk - c[i];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) px[v] += a[i][v] * b;
+          pzk *= X[2];
+        }
+      }
+      pyk *= X[1];
+    }
+    pxk *= X[0];
+  }
+}
+
+template <int DIM>
+constexpr int dof_of(int deg) {
+  return DIM == 2 ? (deg + 1) * (deg + 2) / 2 : (deg + 1) * (deg + 2) * (deg + 3) / 6;
+}
+
+template <int DIM>
+constexpr int deg_of(int dof) {
+  int d = 0;
+  while (dof_of<DIM>(d + 1) <= dof) ++d;
+  return d;
+}
+
+// ---- ideal gas / fluxes -------------------------------------------------------
+// Rotated 2D states carry 4 variables (rho, m_n, m_t, E): the t2 component of
+// a 2D state is identically zero and stays zero through HLLC.
+template <int NV>
+ZFV_HD double pressure_of(const double *u, double gamma) {
+  double m2 = u[1] * u[1] + u[2] * u[2];
+  if (NV == 5) m2 += u[3] * u[3];
+  double kin = 0.5 * m2 / u[0];
+  return (u[NV - 1] - kin) * (gamma - 1.0);
+}
+
+template <int NV>
+ZFV_HD void euler_flux(const double *u, double p, double *f) {
+  double v = u[1] / u[0];
+  f[0] = u[1];
+  f[1] = v * u[1] + p;
+  f[2] = v * u[2];
+  if (NV == 5) f[3] = v * u[3];
+  f[NV - 1] = v * (u[NV - 1] + p);
+}
+
+template <int NV>
+ZFV_HD void hllc(const double *uL, const double *uR, double gamma, double *nf) {
+  double pL = pressure_of<NV>(uL, gamma), aL = sqrt(gamma * pL / uL[0]);
+  double pR = pressure_of<NV>(uR, gamma), aR = sqrt(gamma * pR / uR[0]);
+  double roe_ratio = sqrt(uR[0] / uL[0]);
+  double inv1 = 1.0 + roe_ratio;
+  double vL = uL[1] / uL[0];
+  double vR = uR[1] / uR[0];
+  double v_tilda = (vL + vR * roe_ratio) / inv1;
+  double HL = (uL[NV - 1] + pL) / uL[0];
+  double HR = (uR[NV - 1] + pR) / uR[0];
+  double H_tilda = (HL + HR * roe_ratio) / inv1;
+  double r1 = (uL[1] / uL[0] + (uR[1] / uR[0]) * roe_ratio) / inv1;
+  double r2 = (uL[2] / uL[0] + (uR[2] / uR[0]) * roe_ratio) / inv1;
+  double vroe = r1 * r1 + r2 * r2;
+  if (NV == 5) {
+    double r3 = (uL[3] / uL[0] + (uR[3] / uR[0]) * roe_ratio) / inv1;
+    vroe += r3 * r3;
+  }
+  double a_tilda = sqrt((gamma - 1.0) * (H_tilda - 0.5 * vroe));
+  double sL = dmin(vL - aL, v_tilda - a_tilda);
+  double sR = dmax(vR + aR, v_tilda + a_tilda);
+  double s_star = (uR[1] * (sR - vR) - uL[1] * (sL - vL) + pL - pR) / (uR[0] * (sR - vR) - uL[0] * (sL - vL));
+  const bool left = 0.0 <= s_star;
+  const double *uK = left ? uL : uR;
+  double pK = left ? pL : pR;
+  euler_flux<NV>(uK, pK, nf);
+  if (sL < 0.0 && 0.0 <= sR) {
+    double sK = left ? sL : sR;
+    double vK = left ? vL : vR;
+    double cK = (sK - vK) / (sK - s_star);
+    nf[0] += sK * (cK * uK[0] - uK[0]);
+    nf[1] += sK * (cK * uK[0] * s_star - uK[1]);
+    nf[2] += sK * (cK * uK[2] - uK[2]);
+    if (NV == 5) nf[3] += sK * (cK * uK[3] - uK[3]);
+    nf[NV - 1] += sK * (cK * (uK[NV - 1] + (s_star - vK) * (uK[0] * s_star + pK / (sK - vK))) - uK[NV - 1]);
+  }
+}
+
+template <int NV>
+ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f) {
+  double pL = pressure_of<NV>(uL, gamma), aL = sqrt(gamma * pL / uL[0]);
+  double pR = pressure_of<NV>(uR, gamma), aR = sqrt(gamma * pR / uR[0]);
+  double fL[NV], fR[NV];
+  euler_flux<NV>(uL, pL, fL);
+  euler_flux<NV>(uR, pR, fR);
+  double s = dmax(fabs(uL[1] / uL[0]) + aL, fabs(uR[1] / uR[0]) + aR);
+  for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fL[v] + fR[v]) - 0.5 * s * (uR[v] - uL[v]);
+}
+
+// ---- the per-cell reconstruction --------------------------------------------------
+//
+// Reads the cell's stencil, forms every stencil polynomial with the
+// precomputed pseudo-inverses, applies the CWENO correction and the WENO
+// weights, then evaluates the result at the quadrature points of the cell's
+// interior faces (rotated into each face's frame) and, for gravity, computes
+// the cell's source term.
+//
+// U: state, nvars x n SoA.  fpv: [side][q][v] x ne.  src: nvars x n or null.
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+ZFV_HD void reconstruct_cell(const Img &m, int64_t i, const double *__restrict__ U, double *__restrict__ fpv,
+                             double *__restrict__ src) {
+  constexpr int NV = DIM + 2;
+  constexpr int NF = DIM + 1;
+  constexpr int NC = DOFC - 1;   // central coefficients beyond the constant
+  constexpr int NBC = DIM;       // one-sided (linear) coefficients
+  constexpr int DEGC = deg_of<DIM>(DOFC);
+  const int64_t n = m.n;
+  const double gamma = m.gamma;
+
+  // own state and scale (include/zisa/model/characteristic_scale.hpp:23-32)
+  double u0[NV];
+  for (int v = 0; v < NV; ++v) u0[v] = U[v * n + i];
+  double msq = u0[1] * u0[1] + u0[2] * u0[2];
+  if (DIM == 3) msq += u0[3] * u0[3];
+  const double rho0 = u0[0];
+  const double Eint0 = u0[NV - 1] - 0.5 * msq / rho0;
+  const double cs = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
+  double scale[NV];
+  scale[0] = rho0;
+  for (int d = 0; d < DIM; ++d) scale[1 + d] = cs;
+  scale[NV - 1] = Eint0;
+  double inv[NV];
+  for (int v = 0; v < NV; ++v) inv[v] = 1.0 / scale[v];
+
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+
+  // local equilibrium of this cell
+  double th[2] = {0.0, 0.0};
+  double phi_ref = 0.0;
+  const bool wb = m.eq != 0;
+  bool eq_found = false;
+  if (wb) {
+    double xref[3];
+    volume_point<DIM>(m, 0, vtx, xref);
+    phi_ref = phi<DIM>(m, xref);
+    double vol_i = m.vol[i];
+    // cell average of the equilibrium with parameters t over this cell
+    auto cell_avg = [&](const double *t, double &ar, double &aE) {
+      double sr = 0.0, sE = 0.0;
+      for (int q = 0; q < m.qv; ++q) {
+        double x[3];
+        volume_point<DIM>(m, q, vtx, x);
+        double r, E;
+        eq_eval<DIM>(m, t, phi_ref, x, r, E);
+        double w = vol_i * m.vw[q];
+        if (q == 0) {
+          sr = w * r;
+          sE = w * E;
+        } else {
+          sr = sr + w * r;
+          sE = sE + w * E;
+        }
+      }
+      ar = (1.0 * sr) / vol_i;
+      aE = (1.0 * sE) / vol_i;
+    };
+    if (m.eq == 2) {
+      double RT = (gamma - 1.0) * Eint0 / rho0;
+      double t1[2] = {1.0, RT};
+      double ar, aE;
+      cell_avg(t1, ar, aE);
+      th[0] = rho0 / ar;
+      th[1] = RT;
+      eq_found = true;
+    } else {
+      // quasi-Newton with a central finite-difference Jacobian
+      // (include/zisa/math/quasi_newton.hpp:12-50)
+      double p = Eint0 * (gamma - 1.0);
+      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / pow(rho0, gamma)};
+      double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
+      double x[2] = {guess[0], guess[1]};
+      auto f = [&](const double *t, double *fo) {
+        double ar, aE;
+        cell_avg(t, ar, aE);
+        fo[0] = rho0 - ar;
+        fo[1] = Eint0 - aE;
+      };
+      double fx[2];
+      f(x, fx);
+      double dx[2] = {2.0 * atol[0] + 1.0, 2.0 * atol[1] + 1.0};
+      double h0[2] = {0, 0}, h1[2] = {0, 0};  // previous two steps
+      bool ok = true;
+      int iter = 0;
+      while (!(fabs(dx[0]) <= atol[0] && fabs(dx[1]) <= atol[1]) && iter < 20) {
+        double J[2][2];
+        for (int dir = 0; dir < 2; ++dir) {
+          double e = 1e-6 * fabs(x[dir]);
+          double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]};
+          xp[dir] = x[dir] + 0.5 * e;
+          xm[dir] = x[dir] - 0.5 * e;
+          double fp[2], fm[2];
+          f(xp, fp);
+          f(xm, fm);
+          J[dir][0] = (fp[0] - fm[0]) / e;
+          J[dir][1] = (fp[1] - fm[1]) / e;
+        }
+        double inv_det = 1.0 / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+        dx[0] = inv_det * (J[1][1] * fx[0] - J[1][0] * fx[1]);
+        dx[1] = inv_det * (-J[0][1] * fx[0] + J[0][0] * fx[1]);
+        x[0] -= dx[0];
+        x[1] -= dx[1];
+        f(x, fx);
+        if (iter >= 4 && !(dx[0] <= atol[0] && dx[1] <= atol[1])) {
+          for (int k = 0; k < 2; ++k) {
+            double rate = log(fabs(dx[k]) / fabs(h1[k])) / log(fabs(h1[k]) / fabs(h0[k]));
+            if (!(rate >= 0.0)) ok = false;
+          }
+        }
+        h0[0] = h1[0];
+        h0[1] = h1[1];
+        h1[0] = dx[0];
+        h1[1] = dx[1];
+        if (!ok) break;
+        ++iter;
+      }
+      if (ok && !(iter == 20 && !(fabs(dx[0]) <= 1000.0 * atol[0] && fabs(dx[1]) <= 1000.0 * atol[1]))) {
+        th[0] = x[0];
+        th[1] = x[1];
+        eq_found = true;
+      }
+    }
+  }
+
+  // equilibrium average of this cell's equilibrium over cell j (shifted)
+  auto member_eq = [&](int64_t j, int code, double &er, double &eE) {
+    er = 0.0;
+    eE = 0.0;
+    if (!wb || !eq_found) return;
+    double w3[DIM + 1][3];
+    load_corners<DIM>(m, j, w3);
+    for (int k = 0; k < DIM + 1; ++k)
+      for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
+    double vol_j = m.vol[j];
+    double sr = 0.0, sE = 0.0;
+    for (int q = 0; q < m.qv; ++q) {
+      double x[3];
+      volume_point<DIM>(m, q, w3, x);
+      double r, E;
+      eq_eval<DIM>(m, th, phi_ref, x, r, E);
+      double w = vol_j * m.vw[q];
+      if (q == 0) {
+        sr = w * r;
+        sE = w * E;
+      } else {
+        sr = sr + w * r;
+        sE = sE + w * E;
+      }
+    }
+    er = (1.0 * sr) / vol_j;
+    eE = (1.0 * sE) / vol_j;
+  };
+
+  double q0[NV];
+  {
+    double er, eE;
+    member_eq(i, 0, er, eE);
+    for (int v = 0; v < NV; ++v) q0[v] = u0[v];
+    q0[0] -= er;
+    q0[NV - 1] -= eE;
+    for (int v = 0; v < NV; ++v) q0[v] = q0[v] * inv[v];
+  }
+
+  const int32_t meta = m.meta[i];
+  const bool first_order = (meta & 8) != 0;
+  const int kh = meta & 7;
+
+  // polynomial coefficients: central c[alpha][v] (alpha = 1..DOFC-1),
+  // one-sided b[k][alpha][v] (alpha = 1..DIM); constants a0 per stencil.
+  double c[NC][NV];
+  double b[NB > 0 ? NB : 1][NBC][NV];
+  for (int a = 0; a < NC; ++a)
+    for (int v = 0; v < NV; ++v) c[a][v] = 0.0;
+  for (int k = 0; k < NB; ++k)
+    for (int a = 0; a < NBC; ++a)
+      for (int v = 0; v < NV; ++v) b[k][a][v] = 0.0;
+
+  if (!first_order) {
+    // central stencil: rows 0 .. RC-1
+#pragma unroll 2
+    for (int r = 0; r < RC; ++r) {
+      const int32_t rw = m.rows[r * n + i];
+      const int64_t j = rw & ROW_CELL_MASK;
+      double d[NV];
+      for (int v = 0; v < NV; ++v) d[v] = U[v * n + j];
+      if (wb) {
+        double er, eE;
+        member_eq(j, rw >> ROW_CELL_BITS, er, eE);
+        d[0] -= er;
+        d[NV - 1] -= eE;
+      }
+      for (int v = 0; v < NV; ++v) d[v] = d[v] * inv[v] - q0[v];
+#pragma unroll
+      for (int a = 0; a < NC; ++a) {
+        const double p = m.P[(a * RC + r) * n + i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) c[a][v] += p * d[v];
+      }
+    }
+    // one-sided stencils
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int off = NC * RC + k * NBC * RB;
+      const int roff = RC + k * RB;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int32_t rw = m.rows[(roff + r) * n + i];
+        const int64_t j = rw & ROW_CELL_MASK;
+        double d[NV];
+        for (int v = 0; v < NV; ++v) d[v] = U[v * n + j];
+        if (wb) {
+          double er, eE;
+          member_eq(j, rw >> ROW_CELL_BITS, er, eE);
+          d[0] -= er;
+          d[NV - 1] -= eE;
+        }
+        for (int v = 0; v < NV; ++v) d[v] = d[v] * inv[v] - q0[v];
+#pragma unroll
+        for (int a = 0; a < NBC; ++a) {
+          const double p = m.P[(off + a * RB + r) * n + i];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) b[k][a][v] += p * d[v];
+        }
+      }
+    }
+  }
+
+  // final polynomial a[alpha][v], alpha < DOFC
+  double a[DOFC][NV];
+  if (first_order) {
+    // one constant stencil with weight 1 (make_reconstruction_array o1 params)
+    for (int v = 0; v < NV; ++v) {
+      // p = 0 + (al/al) * p0, p0 = (p0)/1 for CWENO
+      double w = 1.0;
+      a[0][v] = 0.0 + w * q0[v];
+    }
+    for (int al = 1; al < DOFC; ++al)
+      for (int v = 0; v < NV; ++v) a[al][v] = 0.0;
+  } else {
+    double a0[NB + 1][NV];
+    for (int k = 0; k <= NB; ++k)
+      for (int v = 0; v < NV; ++v) a0[k][v] = q0[v];
+    if (m.mode == 0) {
+      // CWENO: p_hi <- (p_hi - sum_{k != hi} c_k p_k) * (1 / c_hi)
+      if (kh == 0) {
+        for (int k = 1; k <= NB; ++k) {
+          const double w = m.lw[k];
+          for (int v = 0; v < NV; ++v) {
+            a0[0][v] = a0[0][v] - w * a0[k][v];
+            for (int al = 0; al < NBC; ++al) c[al][v] = c[al][v] - w * b[k - 1][al][v];
+          }
+        }
+        const double iw = 1.0 / m.lw[0];
+        for (int v = 0; v < NV; ++v) {
+          a0[0][v] = iw * a0[0][v];
+          for (int al = 0; al < NC; ++al) c[al][v] = iw * c[al][v];
+        }
+      } else if (NB > 0) {
+        // the corrected polynomial is a one-sided one (central stencil reduced to order 1)
+        const int hb = kh - 1;
+        for (int k = 0; k <= NB; ++k) {
+          if (k == kh) continue;
+          const double w = m.lw[k];
+          for (int v = 0; v < NV; ++v) {
+            a0[kh][v] = a0[kh][v] - w * a0[k][v];
+            for (int al = 0; al < NBC; ++al) {
+              double o = (k == 0) ? c[al][v] : b[k - 1][al][v];
+              b[hb][al][v] = b[hb][al][v] - w * o;
+            }
+          }
+        }
+        const double iw = 1.0 / m.lw[kh];
+        for (int v = 0; v < NV; ++v) {
+          a0[kh][v] = iw * a0[kh][v];
+          for (int al = 0; al < NBC; ++al) b[hb][al][v] = iw * b[hb][al][v];
+        }
+      }
+    }
+    // smoothness indicators and nonlinear weights (hybrid_weno.cpp:110-127)
+    double alpha[NB + 1];
+    double al_tot = 0.0;
+    {
+      double IS = 0.0;
+      for (int v = 0; v < NV; ++v) {
+        double s = 0.0;
+        for (int al = 0; al < NC; ++al) s += c[al][v] * c[al][v];
+        IS = v == 0 ? s : dmax(IS, s);
+      }
+      double isp = m.exponent == 4.0 ? sqr(sqr(IS)) : pow(IS, m.exponent);
+      alpha[0] = m.lw[0] / (m.eps + isp);
+      al_tot += alpha[0];
+    }
+    for (int k = 0; k < NB; ++k) {
+      double IS = 0.0;
+      for (int v = 0; v < NV; ++v) {
+        double s = 0.0;
+        for (int al = 0; al < NBC; ++al) s += b[k][al][v] * b[k][al][v];
+        IS = v == 0 ? s : dmax(IS, s);
+      }
+      double isp = m.exponent == 4.0 ? sqr(sqr(IS)) : pow(IS, m.exponent);
+      alpha[k + 1] = m.lw[k + 1] / (m.eps + isp);
+      al_tot += alpha[k + 1];
+    }
+    for (int v = 0; v < NV; ++v) {
+      const double w0 = alpha[0] / al_tot;
+      a[0][v] = 0.0 + w0 * a0[0][v];
+      for (int al = 1; al < DOFC; ++al) a[al][v] = 0.0 + w0 * c[al - 1][v];
+      for (int k = 0; k < NB; ++k) {
+        const double w = alpha[k + 1] / al_tot;
+        a[0][v] = a[0][v] + w * a0[k + 1][v];
+        for (int al = 1; al <= NBC; ++al) a[al][v] = a[al][v] + w * b[k][al - 1][v];
+      }
+    }
+  }
+
+  // moments: zero for alpha < 3
+  double cm[DOFC];
+  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
+  double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
+  const double len = m.len[i];
+  const uint32_t fperm = m.fperm[i];
+
+  // scaled reconstruction (+ equilibrium background) at x
+  auto eval = [&](const double *x, double *u, double *delta) {
+    double X[3] = {(x[0] - xc[0]) / len, (x[1] - xc[1]) / len, DIM == 3 ? (x[2] - xc[2]) / len : 0.0};
+    double px[NV];
+    poly_eval<DIM, DEGC, NV>(a, cm, X, px);
+    for (int v = 0; v < NV; ++v) delta[v] = scale[v] * px[v];
+    for (int v = 0; v < NV; ++v) u[v] = delta[v];
+    if (wb && eq_found) {
+      double r, E;
+      eq_eval<DIM>(m, th, phi_ref, x, r, E);
+      u[0] = r + delta[0];
+      for (int d = 0; d < DIM; ++d) u[1 + d] = 0.0 + delta[1 + d];
+      u[NV - 1] = E + delta[NV - 1];
+    }
+  };
+
+  double s_acc[NV];
+  for (int v = 0; v < NV; ++v) s_acc[v] = 0.0;
+  const bool gravity_src = (m.gravity != 0) && (src != nullptr);
+
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int32_t cf = m.cface[f * n + i];
+    double w3[3][3];
+    face_vertices<DIM>(fperm, f, vtx, w3);
+    double nn[3], t1[3], t2[3], area;
+    face_frame<DIM>(w3, nn, t1, t2, area);
+    const bool exterior = (cf & 2) != 0;
+    // outward normal sign for the source term (src/zisa/math/face.cpp:24-27)
+    double x0f[3];
+    face_point<DIM>(m, 0, w3, x0f);
+    double sgn = 0.0;
+    if (gravity_src && wb) {
+      double dd = nn[0] * (x0f[0] - xc[0]) + nn[1] * (x0f[1] - xc[1]) + (DIM == 3 ? nn[2] * (x0f[2] - xc[2]) : 0.0);
+      sgn = double((0.0 < dd) - (dd < 0.0));
+    }
+    double fs[NV];
+    for (int v = 0; v < NV; ++v) fs[v] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double x[3];
+      face_point<DIM>(m, q, w3, x);
+      if (!exterior) {
+        double u[NV], dl[NV];
+        eval(x, u, dl);
+        // rotate into the face frame
+        double r[NV];
+        r[0] = u[0];
+        r[NV - 1] = u[NV - 1];
+        if (DIM == 2) {
+          r[1] = u[1] * nn[0] + u[2] * nn[1];
+          r[2] = u[1] * t1[0] + u[2] * t1[1];
+        } else {
+          r[1] = u[1] * nn[0] + u[2] * nn[1] + u[3] * nn[2];
+          r[2] = u[1] * t1[0] + u[2] * t1[1] + u[3] * t1[2];
+          r[3] = u[1] * t2[0] + u[2] * t2[1] + u[3] * t2[2];
+        }
+        const int64_t e = cf >> 2;
+        const int side = cf & 1;
+        for (int v = 0; v < NV; ++v) fpv[((side * Q + q) * NV + v) * m.ne + e] = r[v];
+      }
+      if (gravity_src && wb && eq_found) {
+        double r, E;
+        eq_eval<DIM>(m, th, phi_ref, x, r, E);
+        double p_eq = E * (gamma - 1.0);
+        double w = area * m.fw[q];
+        for (int d = 0; d < DIM; ++d) {
+          double t = p_eq * (sgn * nn[d]);
+          fs[1 + d] = (q == 0) ? w * t : fs[1 + d] + w * t;
+        }
+      }
+    }
+    if (gravity_src && wb)
+      for (int v = 0; v < NV; ++v) s_acc[v] += 1.0 * fs[v];
+  }
+
+  if (gravity_src) {
+    const double vol_i = m.vol[i];
+    double cs_[NV];
+    for (int v = 0; v < NV; ++v) cs_[v] = 0.0;
+    for (int q = 0; q < m.qv; ++q) {
+      double x[3];
+      volume_point<DIM>(m, q, vtx, x);
+      double u[NV], dl[NV];
+      eval(x, u, dl);
+      double gp[3];
+      grad_phi<DIM>(m, x, gp);
+      double drho = wb ? dl[0] : u[0];
+      double sv[NV];
+      sv[0] = 0.0;
+      double mg = 0.0;
+      for (int d = 0; d < DIM; ++d) {
+        sv[1 + d] = -drho * gp[d];
+        mg += u[1 + d] * gp[d];
+      }
+      sv[NV - 1] = -mg;
+      double w = vol_i * m.vw[q];
+      for (int v = 0; v < NV; ++v) cs_[v] = (q == 0) ? w * sv[v] : cs_[v] + w * sv[v];
+    }
+    for (int v = 0; v < NV; ++v) {
+      double sum = wb ? (s_acc[v] + 1.0 * cs_[v]) : 1.0 * cs_[v];
+      src[v * n + i] = sum / vol_i;
+    }
+  }
+}
+
+// ---- the per-face flux ----------------------------------------------------------
+// G[v][e] = sum_q w_q F(uL_q, uR_q) in the face frame, w_q the normalized
+// face weights (the face measure is applied by the cell update).
+template <int DIM, int Q>
+ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, double *__restrict__ G) {
+  constexpr int NV = DIM + 2;
+  const int64_t ne = m.ne;
+  double acc[NV];
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    double uL[NV], uR[NV], f[NV];
+    for (int v = 0; v < NV; ++v) {
+      uL[v] = fpv[((0 * Q + q) * NV + v) * ne + e];
+      uR[v] = fpv[((1 * Q + q) * NV + v) * ne + e];
+    }
+    if (m.flux == 0) hllc<NV>(uL, uR, m.gamma, f);
+    else rusanov<NV>(uL, uR, m.gamma, f);
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + m.fw[q] * f[v];
+  }
+  for (int v = 0; v < NV; ++v) G[v * ne + e] = acc[v];
+}
+
+// ---- the per-cell rate of change -------------------------------------------------
+// L = -(1/vol) sum_faces (+-) |face| R^T G  (+ source) (+ flux BC)
+template <int DIM>
+ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, const double *__restrict__ G,
+                      const double *__restrict__ src, double *L) {
+  constexpr int NV = DIM + 2;
+  constexpr int NF = DIM + 1;
+  const int64_t n = m.n;
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+  const uint32_t fperm = m.fperm[i];
+  double acc[NV];
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int32_t cf = m.cface[f * n + i];
+    const bool exterior = (cf & 2) != 0;
+    if (exterior && m.flux_bc == 0) continue;
+    double w3[3][3];
+    face_vertices<DIM>(fperm, f, vtx, w3);
+    double nn[3], t1[3], t2[3], area;
+    face_frame<DIM>(w3, nn, t1, t2, area);
+    double g[NV];
+    double sgn;
+    if (!exterior) {
+      const int64_t e = cf >> 2;
+      for (int v = 0; v < NV; ++v) g[v] = G[v * m.ne + e];
+      sgn = (cf & 1) ? 1.0 : -1.0;
+    } else {
+      // FluxBC: physical flux of the cell average in the face frame
+      double u[NV], r[NV];
+      for (int v = 0; v < NV; ++v) u[v] = U[v * n + i];
+      r[0] = u[0];
+      r[NV - 1] = u[NV - 1];
+      if (DIM == 2) {
+        r[1] = u[1] * nn[0] + u[2] * nn[1];
+        r[2] = u[1] * t1[0] + u[2] * t1[1];
+      } else {
+        r[1] = u[1] * nn[0] + u[2] * nn[1] + u[3] * nn[2];
+        r[2] = u[1] * t1[0] + u[2] * t1[1] + u[3] * t1[2];
+        r[3] = u[1] * t2[0] + u[2] * t2[1] + u[3] * t2[2];
+      }
+      euler_flux<NV>(r, pressure_of<NV>(r, m.gamma), g);
+      sgn = -1.0;
+      // the flux BC uses the face measure directly (weights sum to one)
+    }
+    // back to world frame
+    double wv[NV];
+    wv[0] = g[0];
+    wv[NV - 1] = g[NV - 1];
+    if (DIM == 2) {
+      wv[1] = g[1] * nn[0] + g[2] * t1[0];
+      wv[2] = g[1] * nn[1] + g[2] * t1[1];
+    } else {
+      wv[1] = g[1] * nn[0] + g[2] * t1[0] + g[3] * t2[0];
+      wv[2] = g[1] * nn[1] + g[2] * t1[1] + g[3] * t2[1];
+      wv[3] = g[1] * nn[2] + g[2] * t1[2] + g[3] * t2[2];
+    }
+    for (int v = 0; v < NV; ++v) acc[v] += sgn * (area * wv[v]);
+  }
+  const double vol = m.vol[i];
+  for (int v = 0; v < NV; ++v) L[v] = acc[v] / vol;
+  if (m.gravity != 0 && src != nullptr)
+    for (int v = 0; v < NV; ++v) L[v] += src[v * n + i];
+}
+
+}  // namespace zfv
