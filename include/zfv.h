@@ -79,7 +79,9 @@ typedef struct zfv_params {
   int32_t edge_deg, volume_deg, moments_deg;  /* quadrature degrees, -1 = max */
   int32_t flux_bc;                /* 0 none, 1 constant-extrapolation FluxBC */
   int32_t steps_per_recompute;    /* must be 1: equilibria are re-solved every call */
-  int32_t reserved;
+  int32_t scaling;                /* 0 EulerScaling (the experiments' choice), 1 UnityScaling
+                                     (include/zisa/model/characteristic_scale.hpp:14-46);
+                                     the kernels implement EulerScaling */
   double recompute_threshold;
 } zfv_params;
 
@@ -155,6 +157,12 @@ typedef struct zfv_kernel_times {
   int32_t n_launches;
 } zfv_kernel_times;
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out);
+
+/* Streaming copy of `bytes` bytes between two device buffers, `iters` times,
+ * timed with HIP events: the measured HBM copy rate (read + write bytes per
+ * second) used to calibrate the roofline and the PMC byte counters.  Has no
+ * counterpart in the reference. */
+int zfv_bandwidth_probe(int64_t bytes, int32_t iters, double *gbytes_per_s);
 
 const char *zfv_last_error(void);
 

@@ -42,6 +42,7 @@ static Params to_params(const zfv_params *p) {
   q.flux_bc = p->flux_bc;
   q.steps_per_recompute = p->steps_per_recompute;
   q.recompute_threshold = p->recompute_threshold;
+  q.scaling = p->scaling;
   return q;
 }
 

@@ -233,7 +233,16 @@ struct LocalEquilibrium {
             if (!(rate >= 0.0)) ok = false;
           }
         }
-        if (!ok) throw Error("quasi_newton: Is not converging.");
+        if (!ok) {
+          std::string msg = "quasi_newton: Is not converging. guess=(" + std::to_string(guess[0]) + "," +
+                            std::to_string(guess[1]) + ") dx:";
+          char buf[64];
+          for (const auto &h : hist) {
+            snprintf(buf, sizeof(buf), " (%.3e,%.3e)", h[0], h[1]);
+            msg += buf;
+          }
+          throw Error(msg);
+        }
       }
       ++iter;
     }
@@ -346,6 +355,7 @@ struct LocalRC {
   std::vector<double> linear_weights;
   double epsilon = 1e-6, exponent = 4.0;
   int mode = 0;  // 0 CWENO-AO, 1 WENO-AO
+  int scaling = 0;  // 0 EulerScaling, 1 UnityScaling
   LocalEquilibrium eq;
   Vars scale{0, 0, 0, 0, 0};
   std::vector<RhoE> rhoEbar;
@@ -406,6 +416,7 @@ Solver::Solver(std::shared_ptr<Grid> grid, const Params &params,
     r.epsilon = params.epsilon;
     r.exponent = params.exponent;
     r.mode = params.reconstruction;
+    r.scaling = params.scaling;
     r.steps_per_recompute = params.steps_per_recompute;
     r.recompute_threshold = params.recompute_threshold;
     const bool first_order = (r.fam->stencils.size() == 1 && r.fam->order == 1);
@@ -459,12 +470,17 @@ static void compute_local(const Grid &g, const EOS &eos, LocalRC &r, std::vector
   auto compute_equilibrium = [&]() {
     const Vars &u0 = u_local[0];
     RhoE self{u0[0], internal_energy(u0)};
-    // EulerScaling (include/zisa/model/characteristic_scale.hpp:23-32)
-    if (self.rho <= 0.0) throw Error("Invalid density.");
-    if (self.E <= 0.0) throw Error("Invalid energy.");
-    double p = self.E * (eos.gamma - 1.0);
-    double cs = std::sqrt(eos.gamma * p / self.rho);
-    r.scale = {self.rho, cs, cs, cs, self.E};
+    if (r.scaling == 1) {
+      // UnityScaling (include/zisa/model/characteristic_scale.hpp:39-46)
+      r.scale = {1.0, 1.0, 1.0, 1.0, 1.0};
+    } else {
+      // EulerScaling (include/zisa/model/characteristic_scale.hpp:23-32)
+      if (self.rho <= 0.0) throw Error("Invalid density.");
+      if (self.E <= 0.0) throw Error("Invalid energy.");
+      double p = self.E * (eos.gamma - 1.0);
+      double cs = std::sqrt(eos.gamma * p / self.rho);
+      r.scale = {self.rho, cs, cs, cs, self.E};
+    }
     r.eq.solve(self, g.cells[r.i]);
     for (size_t il = 0; il < l2g.size(); ++il) {
       // the equilibrium of cell i averaged over stencil member il (in i's frame)
@@ -600,14 +616,21 @@ void Solver::rate(const double *u, double *dudt, double /*t*/) {
   // (include/zisa/fvm_loops/flux_loop.hpp:96-195)
   auto patch = [&](const std::vector<int> &cells, const std::vector<int> &edges) {
     // EulerGlobalReconstruction::compute (global_reconstruction_impl.hpp:133-163)
+    std::string err;
 #pragma omp parallel for schedule(static, 8) if (im.n_threads > 1)
     for (size_t ii = 0; ii < cells.size(); ++ii) {
       int i = cells[ii];
       LocalRC &r = im.rc[i];
       std::vector<Vars> u_local(r.fam->l2g.size());
       for (size_t q = 0; q < u_local.size(); ++q) u_local[q] = load(u, r.fam->l2g[q].cell);
-      compute_local(g, im.eos, r, u_local);
+      try {
+        compute_local(g, im.eos, r, u_local);
+      } catch (const std::exception &ex) {
+#pragma omp critical
+        if (err.empty()) err = "cell " + std::to_string(i) + ": " + ex.what();
+      }
     }
+    if (!err.empty()) throw Error(err);
 #pragma omp parallel for schedule(static, 8) if (im.n_threads > 1)
     for (size_t ie = 0; ie < edges.size(); ++ie) {
       int e = edges[ie];

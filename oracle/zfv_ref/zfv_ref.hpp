@@ -83,6 +83,7 @@ struct Params {
   int flux_bc = 0;  // 0 none, 1 FluxBC (constant extrapolation)
   int steps_per_recompute = 1;
   double recompute_threshold = 1e300;
+  int scaling = 0;  // 0 EulerScaling, 1 UnityScaling
 };
 
 // ---------------------------------------------------------------------------

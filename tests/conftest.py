@@ -1,0 +1,71 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+
+
+def _has_gpu():
+    return os.path.exists("/dev/kfd")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Skip-free guard: GPU tests must run on a GPU box; fail loudly without one."""
+    if not _has_gpu():
+        pytest.fail("no GPU visible for a test marked gpu")
+    import zisafvm_amd
+
+    return zisafvm_amd.load_library()
+
+
+def rel_err(a, b, mask=None):
+    """Per-variable max |a-b| / max |b| (absolute where b vanishes)."""
+    import numpy as np
+
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    out = []
+    for v in range(b.shape[1]):
+        den = np.abs(b[:, v]).max()
+        diff = np.abs(a[:, v] - b[:, v]).max()
+        out.append(diff / den if den > 0 else diff)
+    return out
+
+
+def pressure_scale(mesh, u, gamma):
+    """max_i sum_faces p_i |face| / vol_i: the size of the pressure terms that
+    cancel in a hydrostatic equilibrium (the normalisation of well-balanced
+    rate-of-change errors)."""
+    import numpy as np
+
+    kin = 0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2 + u[:, 3] ** 2) / u[:, 0]
+    p = (u[:, 4] - kin) * (gamma - 1.0)
+    c = mesh.corners
+    if mesh.n_dims == 2:
+        perim = sum(np.linalg.norm(c[:, (a + 1) % 3] - c[:, a], axis=1) for a in range(3))
+    else:
+        faces = [(0, 1, 3), (0, 2, 1), (0, 3, 2), (1, 2, 3)]
+        perim = sum(0.5 * np.linalg.norm(np.cross(c[:, b] - c[:, a], c[:, d] - c[:, a]), axis=1) for a, b, d in faces)
+    return float((np.abs(p) * perim / mesh.volumes()).max())
+
+
+def wb_err(a, b, mesh, u, gamma, mask=None):
+    """Per-variable max |a-b| / max(max |b|, pressure scale)."""
+    import numpy as np
+
+    if mask is None:
+        mask = mesh.ghost == 0
+    s = pressure_scale(mesh, u, gamma)
+    out = []
+    for v in range(b.shape[1]):
+        den = max(np.abs(b[mask, v]).max(), s)
+        out.append(np.abs(a[mask, v] - b[mask, v]).max() / den)
+    return out

@@ -1,0 +1,106 @@
+"""The C-ABI library loads and exports every function include/zfv.h declares;
+host-only entry points (mesh generation) work without a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import zisafvm_amd as z
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(zfv_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(z.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_device_is_a_clean_error():
+    if os.path.exists("/dev/kfd"):
+        return  # covered by the GPU tests
+    wl_mesh = z.lattice_mesh_2d(6)
+    p = z.make_params()
+    try:
+        z.Solver(wl_mesh, p)
+    except z.ZfvError as e:
+        assert "device" in str(e)
+    else:
+        raise AssertionError("zfv_create succeeded without a device")
+
+
+def test_invalid_configuration_is_rejected_before_device_work():
+    p = z.make_params(order=3)
+    p.steps_per_recompute = 2
+    mesh = z.lattice_mesh_2d(6)
+    try:
+        z.Solver(mesh, p)
+    except z.ZfvError as e:
+        assert "steps_per_recompute" in str(e) or "device" in str(e)
+    else:
+        raise AssertionError("accepted steps_per_recompute = 2")
+
+
+def test_lattice_mesh_counts_and_volumes():
+    m = z.lattice_mesh_2d(16, jitter=0.2, seed=4, periodic=(True, True))
+    assert m.n_cells == 2 * 16 * 16
+    assert abs(m.volumes().sum() - 1.0) < 1e-12
+    assert (m.ghost == 0).all()
+    m2 = z.lattice_mesh_2d(16, jitter=0.2, seed=4, periodic=(True, False), ghost_layers=3)
+    assert m2.n_cells == 2 * 16 * 22
+    interior = m2.ghost == 0
+    assert abs(m2.volumes()[interior].sum() - 1.0) < 1e-12
+
+
+def test_lattice_mesh_is_delaunay():
+    # every interior edge is locally Delaunay: opposite angles sum to <= pi
+    m = z.lattice_mesh_2d(12, jitter=0.25, seed=6, periodic=(False, False), ghost_layers=0)
+    edges = {}
+    for i, tri in enumerate(m.vertex_ids):
+        for k in range(3):
+            key = tuple(sorted((tri[k], tri[(k + 1) % 3])))
+            edges.setdefault(key, []).append((i, (k + 2) % 3))
+    worst = 0.0
+    for key, sides in edges.items():
+        if len(sides) != 2:
+            continue
+        ang = 0.0
+        for i, opp in sides:
+            c = m.corners[i]
+            p, a, b = c[opp], c[(opp + 1) % 3], c[(opp + 2) % 3]
+            u, v = a - p, b - p
+            ang += np.arccos(np.clip(u @ v / np.linalg.norm(u) / np.linalg.norm(v), -1, 1))
+        worst = max(worst, ang)
+    assert worst <= np.pi + 1e-12
+
+
+def test_ball_mesh():
+    m = z.ball_mesh_3d(10, radius=1.0, ghost_radius=0.8, jitter=0.1, seed=2)
+    assert m.n_cells > 0
+    assert (m.volumes() > 0).all()
+    r = np.linalg.norm(m.centroids(), axis=1)
+    assert r.max() <= 1.0
+    assert ((r > 0.8) == (m.ghost == 1)).all()
+    # total volume close to the ball's (jagged boundary)
+    assert abs(m.volumes().sum() / (4.0 / 3.0 * np.pi) - 1.0) < 0.1
+
+
+def test_mesh_generation_is_deterministic():
+    a = z.lattice_mesh_2d(8, seed=42)
+    b = z.lattice_mesh_2d(8, seed=42)
+    np.testing.assert_array_equal(a.corners, b.corners)
+    c = z.lattice_mesh_2d(8, seed=43)
+    assert not np.array_equal(a.corners, c.corners)

@@ -58,7 +58,7 @@ class Params(ctypes.Structure):
         ("moments_deg", ctypes.c_int32),
         ("flux_bc", ctypes.c_int32),
         ("steps_per_recompute", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("scaling", ctypes.c_int32),
         ("recompute_threshold", ctypes.c_double),
     ]
 
@@ -190,6 +190,7 @@ def load_library(path: str = LIB_PATH):
         "zfv_get_stencils": (c_int, [vp, ctypes.c_int32, ip, ctypes.c_int32, ip, ip, ctypes.c_int32]),
         "zfv_time_kernels": (c_int, [vp, ctypes.c_int32, c_double, ctypes.POINTER(KernelTimes)]),
         "zfv_last_error": (ctypes.c_char_p, []),
+        "zfv_bandwidth_probe": (c_int, [ctypes.c_int64, ctypes.c_int32, dp]),
         "zfv_mesh_lattice2d": (
             c_int,
             [ctypes.c_int32, c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -265,6 +266,14 @@ def _mesh_from_buf(lib, buf) -> Mesh:
     gh = np.ctypeslib.as_array(view.ghost_mask, shape=(n,)).copy()
     lib.zfv_mesh_free(buf)
     return Mesh(d, vid, crn, gh)
+
+
+def bandwidth_probe(nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """Measured device copy rate (GB/s, read + write) of a plain streaming copy."""
+    lib = load_library()
+    r = ctypes.c_double()
+    _check(lib.zfv_bandwidth_probe(nbytes, iters, ctypes.byref(r)))
+    return r.value
 
 
 def lattice_mesh_2d(n: int, jitter: float = 0.2, seed: int = 1, periodic=(True, True), ghost_layers: int = 3) -> Mesh:

@@ -124,6 +124,13 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// plain streaming copy (16-byte vectors, grid-stride) for bandwidth calibration
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                                    int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
 // ---------------------------------------------------------------------------
 // launch plan for one shape
 
@@ -585,6 +592,39 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   *out = t;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_bandwidth_probe(int64_t bytes, int32_t iters, double *gbytes_per_s) {
+  ZFV_API_BEGIN
+  check_device();
+  if (bytes < 16 || iters < 1) throw ZfvError(ZFV_ERR_INVALID, "bandwidth probe: bad size");
+  const int64_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HIP_CHECK(hipMalloc(&a, n * 16));
+  HIP_CHECK(hipMalloc(&b, n * 16));
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipMemsetAsync(a, 0, n * 16, s));
+  const unsigned grid = 256 * 8 * 4;  // a few waves per CU on every XCD
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+  HIP_CHECK(hipEventRecord(e0, s));
+  for (int k = 0; k < iters; ++k)
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *gbytes_per_s = 2.0 * double(n) * 16.0 * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return ZFV_OK;
   ZFV_API_END
 }

@@ -841,6 +841,7 @@ Shape shape_for(const zfv_params &p) {
   s.nmom = std::max(0, s.dofc - 3);
   s.q = int((s.dim == 2 ? gl_edge(p.edge_deg) : dunavant(p.edge_deg)).w.size());
   s.qv = int(volume_rule(s.dim, p.volume_deg).w.size());
+  if (p.scaling != 0) throw ZfvError(ZFV_ERR_INVALID, "the kernels implement EulerScaling only (scaling = 0)");
   if (p.steps_per_recompute != 1)
     throw ZfvError(ZFV_ERR_INVALID, "steps_per_recompute must be 1: the kernels re-solve equilibria every call");
   if (p.equilibrium != ZFV_EQ_NONE && p.gravity == ZFV_GRAVITY_NONE)

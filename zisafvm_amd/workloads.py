@@ -28,20 +28,44 @@ from . import (
     make_params,
 )
 
-# volume rules (barycentric, weights) — Dunavant degree 4 / Shunn-Ham degree 3
-_TRI4 = []
-for w, (a, b) in ((0.109951743655322, (0.816847572980459, 0.091576213509771)),
-                  (0.223381589678011, (0.108103018168070, 0.445948490915965))):
-    _TRI4 += [(w, (a, b, b)), (w, (b, a, b)), (w, (b, b, a))]
-_a, _b, _c, _d = 0.7784952948213300, 0.0738349017262234, 0.4062443438840510, 0.0937556561159491
-_TET3 = [(0.0476331348432089, p) for p in ((_a, _b, _b, _b), (_b, _a, _b, _b), (_b, _b, _a, _b), (_b, _b, _b, _a))]
-_TET3 += [(0.1349112434378610, p) for p in ((_c, _c, _d, _d), (_c, _d, _c, _d), (_c, _d, _d, _c),
-                                              (_d, _c, _c, _d), (_d, _c, _d, _c), (_d, _d, _c, _c))]
+# Cell rules (weight, barycentric point): Dunavant degrees 1-5 and Shunn-Ham
+# degrees 1-3, the tables of src/zisa/math/{triangular,tetrahedral}_rule.cpp.
+def _perm3(w, lam):
+    if len(lam) == 1:
+        return [(w, (lam[0],) * 3)]
+    a, b = lam
+    return [(w, (a, b, b)), (w, (b, a, b)), (w, (b, b, a))]
 
 
-def cell_average(mesh: Mesh, f) -> np.ndarray:
-    """Average of f(x) (x: (n,3) -> (n,5)) over every cell with the cell rule."""
-    rule = _TRI4 if mesh.n_dims == 2 else _TET3
+TRI_RULES = {
+    1: _perm3(1.0, [1.0 / 3.0]),
+    2: _perm3(1.0 / 3.0, [2.0 / 3.0, 1.0 / 6.0]),
+    3: _perm3(-0.5625, [1.0 / 3.0]) + _perm3(1.5625 / 3.0, [0.6, 0.2]),
+    4: _perm3(0.109951743655322, [0.816847572980459, 0.091576213509771])
+    + _perm3(0.223381589678011, [0.108103018168070, 0.445948490915965]),
+    5: _perm3(0.225, [1.0 / 3.0]) + _perm3(0.132394152788506, [0.059715871789770, 0.470142064105115])
+    + _perm3(0.125939180544827, [0.797426985353087, 0.101286507323456]),
+}
+_a, _b = 0.5854101966249680, 0.1381966011250110
+_c, _d, _e, _f = 0.7784952948213300, 0.0738349017262234, 0.4062443438840510, 0.0937556561159491
+TET_RULES = {
+    1: [(1.0, (0.25, 0.25, 0.25, 0.25))],
+    2: [(0.25, p) for p in ((_a, _b, _b, _b), (_b, _a, _b, _b), (_b, _b, _a, _b), (_b, _b, _b, _a))],
+    3: [(0.0476331348432089, p) for p in ((_c, _d, _d, _d), (_d, _c, _d, _d), (_d, _d, _c, _d), (_d, _d, _d, _c))]
+    + [(0.1349112434378610, p) for p in ((_e, _e, _f, _f), (_e, _f, _e, _f), (_e, _f, _f, _e),
+                                          (_f, _e, _e, _f), (_f, _e, _f, _e), (_f, _f, _e, _e))],
+}
+_TRI4 = TRI_RULES[4]
+_TET3 = TET_RULES[3]
+
+
+def cell_average(mesh: Mesh, f, deg: Optional[int] = None) -> np.ndarray:
+    """Average of f(x) (x: (n,3) -> (n,5)) over every cell with the cell rule
+    of degree `deg` (default: 4 in 2D, 3 in 3D — the workloads' volume rule)."""
+    if mesh.n_dims == 2:
+        rule = TRI_RULES[deg or 4]
+    else:
+        rule = TET_RULES[deg or 3]
     acc = None
     for w, lam in rule:
         x = sum(l * mesh.corners[:, k, :] for k, l in enumerate(lam))
