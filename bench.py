@@ -3,12 +3,21 @@ one MI355X.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
+Default workload: the 2D well-balanced configuration — isothermal atmosphere
+rho = p = exp(-y) with a 1e-6 Gaussian pressure bump on a seeded jittered
+360 x 360 Delaunay lattice (259,200 triangles inside the unit square plus
+frozen ghost layers), CWENO-AO order 4, HLLC, SSP3, CFL 0.45.
+
 One step = one SSP3 step (3 stages) of the whole instance, replayed from a
-captured HIP graph with the state resident in HBM.  Cell-stage updates/s =
-n_cells * 3 * K / T.  Prints one JSON line (see README of the contract in the
-task description): value, roofline of the dominant kernel (reconstruction,
-HIP-event timed on the handle's stream) and the CPU baseline (the oracle, a
-restatement of the reference's OpenMP CPU path, timed on this host).
+captured HIP graph with the state resident in HBM.  value = n_cells * 3 * K / T
+over exactly K timed steps.  The JSON line also carries
+  - roofline: the dominant kernel (k_reconstruct), algorithmic bytes per launch
+    / its average HIP-event duration on the handle's stream;
+  - batched_vs_plain: the batched configuration (64 members, 5k-200k cells,
+    order 3, per-member CFL every step, one graph per step for all members)
+    against the same members stepped one after another on the plain path;
+  - cpu_baseline: the oracle (the reference's OpenMP CPU path restated) timed
+    on this host's cores over a bounded sample.
 """
 from __future__ import annotations
 
@@ -36,31 +45,28 @@ def cpu_cores():
     return max(1, min(n, 16))
 
 
-def build_workload(name, args):
+def build_workload(name, n):
     from zisafvm_amd import workloads as W
 
     if name == "vortex2d":
-        return W.vortex_2d(n=args.n or 128)
+        return W.vortex_2d(n=n or 128)
     if name == "isothermal2d":
-        return W.isothermal_2d(n=args.n or 360)
+        return W.isothermal_2d(n=n or 360)
     if name == "polytrope3d":
-        return W.polytrope_3d(n=args.n or 40)
+        return W.polytrope_3d(n=n or 40)
     raise SystemExit(f"unknown workload {name}")
 
 
 def cpu_baseline(wl, budget_s):
-    """Oracle RoC calls on this host's cores for about budget_s seconds."""
+    """Oracle rate-of-change calls on this host's cores for about budget_s."""
     from oracle.zfv_ref import Oracle
 
     cores = cpu_cores()
     o = Oracle(wl.mesh, wl.params, threads=cores)
-    if wl.steady is not None:
-        o.set_steady_state(wl.steady)
     t0 = time.perf_counter()
     o.rate(wl.u0)
     first = time.perf_counter() - t0
-    calls = max(1, int(budget_s / max(first, 1e-3)))
-    calls = min(calls, 20)
+    calls = int(min(20, max(1, budget_s / max(first, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(calls):
         o.rate(wl.u0)
@@ -71,26 +77,98 @@ def cpu_baseline(wl, budget_s):
         "unit": "cell-stage updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{calls} rate-of-change evaluations (one RK stage each) of the same {wl.name} instance "
-        f"({n} cells), oracle with OpenMP on {cores} threads, {el:.1f} s",
+        "sample": f"{calls} rate-of-change evaluations (one RK stage each) of the {wl.name} instance "
+        f"({n} cells) by the oracle, OpenMP on {cores} threads, {el:.1f} s",
+    }
+
+
+def run_single(z, wl, steps, warmup):
+    t0 = time.perf_counter()
+    s = z.Solver(wl.mesh, wl.params)
+    t_setup = time.perf_counter() - t0
+    s.set_state(wl.u0)
+    if wl.steady is not None:
+        s.set_steady_state(wl.steady)
+    info = s.info()
+    dt = s.cfl_dt(wl.cfl)
+    s.steps(warmup, 0.0, dt)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.steps(steps, 0.0, dt)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    return s, info, dt, el, t_setup
+
+
+def batched_vs_plain(z, steps):
+    """64 members of the batched configuration: one graph per step for all of
+    them, versus the same members one after another on the plain path."""
+    from zisafvm_amd import workloads as W
+
+    wls = W.batched_2d()
+    cells = sum(w.mesh.n_cells for w in wls)
+    t0 = time.perf_counter()
+    solvers = []
+    for w in wls:
+        s = z.Solver(w.mesh, w.params)
+        s.set_state(w.u0)
+        if w.steady is not None:
+            s.set_steady_state(w.steady)
+        solvers.append(s)
+    t_setup = time.perf_counter() - t0
+    faces = sum(s.info().n_faces for s in solvers)
+    # plain: each member alone, graph-replayed steps with its own CFL dt
+    dts = [s.cfl_dt(w.cfl) for s, w in zip(solvers, wls)]
+    for s, dt in zip(solvers, dts):
+        s.steps(1, 0.0, dt)
+    t0 = time.perf_counter()
+    for s, dt in zip(solvers, dts):
+        s.steps(steps, 0.0, dt)
+    el_plain = time.perf_counter() - t0
+    # batched: all members in one batch, per-member CFL every step
+    for s, w in zip(solvers, wls):
+        s.set_state(w.u0)
+    b = z.Batch(wls[0].params, cell_capacity=cells, face_capacity=faces, max_instances=len(wls))
+    for s, w in zip(solvers, wls):
+        b.add(s, w.cfl, 0.0, 1e9, 0)
+    b.run(1)
+    t0 = time.perf_counter()
+    b.run(steps)
+    el_batch = time.perf_counter() - t0
+    plain = cells * 3 * steps / el_plain
+    batched = cells * 3 * steps / el_batch
+    b.close()
+    return {
+        "members": len(wls),
+        "cells": cells,
+        "steps": steps,
+        "plain_cell_stage_per_s": plain,
+        "batched_cell_stage_per_s": batched,
+        "batched_over_plain": batched / plain,
+        "plain_ms_per_step_all_members": 1e3 * el_plain / steps,
+        "batched_ms_per_step": 1e3 * el_batch / steps,
+        "setup_s": t_setup,
+        "note": "batched includes the per-member CFL reduction every step; plain replays fixed-dt graphs",
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="isothermal2d")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the streaming-copy calibration kernel (1 GiB x 5) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design "
-              f"in DESIGN.md, not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over "
+              f"several GPUs is described in DESIGN.md, and no multi-GPU figure is produced.", file=sys.stderr)
         return 2
 
     import zisafvm_amd as z
@@ -101,38 +179,19 @@ def main():
         return 0
 
     t0 = time.perf_counter()
-    wl = build_workload(args.workload, args)
-    t_mesh = time.perf_counter() - t0
-    log(f"workload {wl.name}: {wl.mesh.n_cells} cells, mesh+IC {t_mesh:.1f}s")
-    t0 = time.perf_counter()
-    s = z.Solver(wl.mesh, wl.params)
-    t_setup = time.perf_counter() - t0
-    s.set_state(wl.u0)
-    if wl.steady is not None:
-        s.set_steady_state(wl.steady)
-    info = s.info()
-    log(f"setup {t_setup:.1f}s, faces {info.n_faces}, device bytes {info.device_bytes/1e9:.2f} GB")
-    dt = s.cfl_dt(wl.cfl)
-    s.steps(args.warmup, 0.0, dt)
-    s.synchronize()
-    t0 = time.perf_counter()
-    s.steps(args.steps, 0.0, dt)
-    s.synchronize()
-    el = time.perf_counter() - t0
+    wl = build_workload(args.workload, args.n)
+    log(f"workload {wl.name}: {wl.mesh.n_cells} cells, mesh+IC {time.perf_counter() - t0:.1f}s")
+    s, info, dt, el, t_setup = run_single(z, wl, args.steps, args.warmup)
     n = wl.mesh.n_cells
-    cell_stages = n * 3 * args.steps
-    value = cell_stages / el
-    ms_per_step = 1e3 * el / args.steps
-    log(f"{args.steps} steps in {el:.3f}s: {value:.3e} cell-stage updates/s")
+    value = n * 3 * args.steps / el
+    log(f"setup {t_setup:.1f}s; {args.steps} steps in {el:.3f}s: {value:.3e} cell-stage updates/s")
 
-    # per-kernel HIP-event timing on the handle's stream
     kt = s.time_kernels(max(2, min(10, args.steps)), dt)
     launches = kt.n_launches
     rec_ms = kt.reconstruct_ms / launches
     face_ms = kt.face_flux_ms / launches
     upd_ms = kt.update_ms / launches
-    rec_bytes = info.recon_bytes_per_cell * n
-    achieved = rec_bytes / (rec_ms * 1e-3) / 1e9
+    achieved = info.recon_bytes_per_cell * n / (rec_ms * 1e-3) / 1e9
     log(f"kernel avg ms: reconstruct {rec_ms:.4f} face {face_ms:.4f} update {upd_ms:.4f}")
 
     out = {
@@ -142,7 +201,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -151,8 +210,11 @@ def main():
         "config": {
             "workload": wl.name,
             "n_cells": n,
+            "n_interior_cells": int((wl.mesh.ghost == 0).sum()),
             "order": wl.meta.get("order"),
             "mesh": wl.meta,
+            "reconstruction": "CWENO-AO",
+            "flux": "HLLC" if wl.params.flux == 0 else "Rusanov",
             "rk": "SSP3",
             "dt": dt,
             "parallelism": "single-gpu",
@@ -171,7 +233,13 @@ def main():
         "setup_s": t_setup,
         "measured_copy_gbs": z.bandwidth_probe(1 << 30, 10),
     }
+    s.close()
+    if not args.no_batch:
+        log("batched vs plain ...")
+        out["batched_vs_plain"] = batched_vs_plain(z, args.batch_steps)
+        log(json.dumps(out["batched_vs_plain"]))
     if not args.no_cpu:
+        log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0

@@ -158,6 +158,32 @@ typedef struct zfv_kernel_times {
 } zfv_kernel_times;
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out);
 
+/* ---- Batches: many instances of one kernel shape advanced together.
+ * Has no counterpart in the reference (which runs one experiment per
+ * process); each member behaves exactly like its own TimeLoop
+ * (src/zisa/fvm_loops/time_loop.cpp:36-75): per step, dt = cfl * LocalCFL
+ * clamped to t_end, one SSP3 step with the frozen BC.  All members of a batch
+ * advance in one captured HIP graph per step (block tables map launched
+ * blocks to members; no per-member launches).  A member's handle must stay
+ * alive while it is in the batch; read its state with zfv_get_state after
+ * zfv_batch_run returns. */
+typedef struct zfv_batch zfv_batch;
+/* Capacity: total cells / interior faces / members the batch can hold. */
+int zfv_batch_create(const zfv_params *shape_params, int64_t cell_capacity, int64_t face_capacity,
+                     int32_t max_instances, zfv_batch **out);
+int zfv_batch_destroy(zfv_batch *b);
+/* Joins at the next step boundary.  The state is the handle's current state
+ * (zfv_set_state).  max_steps <= 0: no step limit. */
+int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_end, int32_t max_steps,
+                  int32_t *slot);
+int zfv_batch_remove(zfv_batch *b, int32_t slot);
+/* n_steps graph replays; members that reach t_end / max_steps stop. */
+int zfv_batch_run(zfv_batch *b, int32_t n_steps);
+int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps_done, int32_t *active);
+/* Block-to-XCD mapping: 1 (default) gives each XCD a contiguous eighth of the
+ * cells so stencil gathers share one L2; 0 is the hardware round robin. */
+int zfv_batch_set_xcd_chunks(zfv_batch *b, int32_t on);
+
 /* Streaming copy of `bytes` bytes between two device buffers, `iters` times,
  * timed with HIP events: the measured HBM copy rate (read + write bytes per
  * second) used to calibrate the roofline and the PMC byte counters.  Has no

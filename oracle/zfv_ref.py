@@ -148,6 +148,18 @@ class Oracle:
         assert w >= 0
         return list(l2g[: size.value]), list(mem[:w])
 
+    def topology(self):
+        n = self.mesh.n_cells
+        k = self.mesh.n_dims + 1
+        n_int, n_ext = self.counts()
+        nbr = np.zeros((n, k), dtype=np.int32)
+        eidx = np.zeros((n, k), dtype=np.int32)
+        normals = np.zeros((n_int + n_ext, 3))
+        ip = ctypes.POINTER(ctypes.c_int32)
+        self.lib.zfv_ref_topology.argtypes = [ctypes.c_void_p, ip, ip, ctypes.POINTER(ctypes.c_double)]
+        self.lib.zfv_ref_topology(self.h, nbr.ctypes.data_as(ip), eidx.ctypes.data_as(ip), _dp(normals))
+        return nbr, eidx, normals
+
 
 def ldlt_solve(M, b):
     M = np.ascontiguousarray(M, dtype=np.float64)
@@ -180,3 +192,11 @@ def rule(kind, deg):
     x = np.zeros(32 * 4)
     n = load().zfv_ref_rule(kind, deg, _dp(w), _dp(x), 32)
     return w[:n].copy(), x[: 4 * n].reshape(n, 4).copy()
+
+
+def rk_ode(kind, dt, t_end):
+    lib = load()
+    lib.zfv_ref_rk_ode.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    out = ctypes.c_double()
+    lib.zfv_ref_rk_ode(kind, dt, t_end, ctypes.byref(out))
+    return out.value

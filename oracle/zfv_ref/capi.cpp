@@ -6,6 +6,7 @@
 #include "../../include/zfv.h"
 #include "zfv_ref.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -128,6 +129,20 @@ int zfv_ref_geometry(void *h, double *vol, double *centers, double *inradii, dou
   return 0;
 }
 
+// topology export: neighbours (-1 invalid) and edge indices per (cell, face),
+// face normals per edge (3 doubles)
+int zfv_ref_topology(void *h, int *nbr, int *edge_idx, double *normals) {
+  const Grid &g = static_cast<Solver *>(h)->grid();
+  for (int i = 0; i < g.n_cells; ++i)
+    for (int k = 0; k < g.max_neighbours; ++k) {
+      nbr[i * g.max_neighbours + k] = g.neighbour(i, k);
+      edge_idx[i * g.max_neighbours + k] = g.edge_indices[size_t(i) * g.max_neighbours + k];
+    }
+  for (int e = 0; e < g.n_edges; ++e)
+    for (int d = 0; d < 3; ++d) normals[3 * e + d] = g.faces[e].normal[d];
+  return 0;
+}
+
 // counts: n_interior_edges, n_exterior_edges
 int zfv_ref_counts(void *h, int *n_int, int *n_ext) {
   const Grid &g = static_cast<Solver *>(h)->grid();
@@ -182,6 +197,24 @@ int zfv_ref_rusanov(double gamma, const double *uL, const double *uR, double *f)
   Vars a{uL[0], uL[1], uL[2], uL[3], uL[4]}, b{uR[0], uR[1], uR[2], uR[3], uR[4]};
   Vars r = rusanov_flux(eos, a, b);
   for (int k = 0; k < 5; ++k) f[k] = r[k];
+  return 0;
+}
+
+// SSP3 on the scalar ODEs of test/zisa/unit_test/ode/runge_kutta.cpp:
+// kind 0: u' = u (u(0) = 1), kind 1: u' = t^2 (u(0) = 0), kind 2: u' = 1.
+// Integrates to t_end with fixed dt, returns u(t_end).
+int zfv_ref_rk_ode(int kind, double dt, double t_end, double *u_out) {
+  double u = (kind == 0) ? 1.0 : 0.0;
+  double t = 0.0;
+  int steps = int(std::lround(t_end / dt));
+  auto f = [kind](const double *x, double *k, double tt) {
+    k[0] += (kind == 0) ? x[0] : (kind == 1 ? tt * tt : 1.0);
+  };
+  for (int s = 0; s < steps; ++s) {
+    rk_step(ssp3_tableau(), 1, &u, t, dt, f, [](double *) {});
+    t += dt;
+  }
+  *u_out = u;
   return 0;
 }
 

@@ -740,36 +740,54 @@ void Solver::rate(const double *u, double *dudt, double /*t*/) {
 // RungeKutta::compute_step + runge_kutta_sum + FrozenBC::apply
 // (src/zisa/ode/runge_kutta.cpp:87-143, 154-165;
 //  src/zisa/boundary/frozen_boundary_condition.cpp:39-55)
+// Butcher tableau "ssp3" (src/zisa/ode/runge_kutta.cpp:154-165)
+const Tableau &ssp3_tableau() {
+  static const Tableau t = {3,
+                            {{0.0, 0.0, 0.0, 0.0}, {1.0, 0.0, 0.0, 0.0}, {0.25, 0.25, 0.0, 0.0}, {0, 0, 0, 0}},
+                            {1.0 / 6, 1.0 / 6, 2.0 / 3, 0.0},
+                            {0.0, 1.0, 0.5, 0.0}};
+  return t;
+}
+
+// RungeKutta::compute_step + runge_kutta_sum
+// (src/zisa/ode/runge_kutta.cpp:87-143): one accumulator per entry, zero
+// coefficients skipped, u_s = u0 + dt * dudt, the BC after every sum.
+void rk_step(const Tableau &tab, size_t n, double *u, double t, double dt,
+             const std::function<void(const double *, double *, double)> &rate_fn,
+             const std::function<void(double *)> &bc_fn) {
+  std::vector<double> u0(u, u + n), ux(n);
+  std::vector<std::vector<double>> k(tab.n_stages, std::vector<double>(n));
+  auto rk_sum = [&](const double *coeffs) {
+    for (size_t q = 0; q < n; ++q) {
+      double dudt = 0.0;
+      for (int s = 0; s < tab.n_stages; ++s)
+        if (coeffs[s] != 0.0) dudt += coeffs[s] * k[s][q];
+      ux[q] = u0[q] + dt * dudt;
+    }
+  };
+  std::fill(k[0].begin(), k[0].end(), 0.0);
+  rate_fn(u0.data(), k[0].data(), t);
+  for (int s = 1; s < tab.n_stages; ++s) {
+    rk_sum(tab.a[s]);
+    bc_fn(ux.data());
+    std::fill(k[s].begin(), k[s].end(), 0.0);
+    rate_fn(ux.data(), k[s].data(), t + tab.c[s] * dt);
+  }
+  rk_sum(tab.b);
+  bc_fn(ux.data());
+  std::copy(ux.begin(), ux.end(), u);
+}
+
 void Solver::step(double *u, double t, double dt) {
   const size_t n = size_t(grid_->n_cells) * 5;
-  static const double a[3][3] = {{0.0, 0.0, 0.0}, {1.0, 0.0, 0.0}, {0.25, 0.25, 0.0}};
-  static const double b[3] = {1.0 / 6, 1.0 / 6, 2.0 / 3};
-  static const double c[3] = {0.0, 1.0, 0.5};
-  std::vector<double> u0(u, u + n), ux(n);
-  std::vector<std::vector<double>> k(3, std::vector<double>(n));
-  auto bc = [&](std::vector<double> &w) {
+  auto bc = [&](double *w) {
+    // FrozenBC::apply (src/zisa/boundary/frozen_boundary_condition.cpp:39-55)
     if (impl_->steady.empty()) return;
     for (int i = 0; i < grid_->n_cells; ++i)
       if (grid_->cell_flags[i].ghost_cell)
         for (int v = 0; v < 5; ++v) w[5 * size_t(i) + v] = impl_->steady[5 * size_t(i) + v];
   };
-  auto rk_sum = [&](const double *coeffs) {
-    for (size_t q = 0; q < n; ++q) {
-      double dudt = 0.0;
-      for (int s = 0; s < 3; ++s)
-        if (coeffs[s] != 0.0) dudt += coeffs[s] * k[s][q];
-      ux[q] = u0[q] + dt * dudt;
-    }
-  };
-  rate(u0.data(), k[0].data(), t);
-  for (int s = 1; s < 3; ++s) {
-    rk_sum(a[s]);
-    bc(ux);
-    rate(ux.data(), k[s].data(), t + c[s] * dt);
-  }
-  rk_sum(b);
-  bc(ux);
-  std::copy(ux.begin(), ux.end(), u);
+  rk_step(ssp3_tableau(), n, u, t, dt, [&](const double *x, double *k, double tt) { rate(x, k, tt); }, bc);
 }
 
 // LocalCFL (include/zisa/model/local_cfl_condition_impl.hpp:25-40)

@@ -223,6 +223,19 @@ Vars hllc_flux(const EOS &eos, const Vars &uL, const Vars &uR);
 Vars rusanov_flux(const EOS &eos, const Vars &uL, const Vars &uR);
 
 // ---------------------------------------------------------------------------
+// Explicit Runge-Kutta in Butcher form (src/zisa/ode/runge_kutta.cpp)
+struct Tableau {
+  int n_stages;
+  double a[4][4];
+  double b[4];
+  double c[4];
+};
+const Tableau &ssp3_tableau();
+void rk_step(const Tableau &tab, size_t n, double *u, double t, double dt,
+             const std::function<void(const double *, double *, double)> &rate_fn,
+             const std::function<void(double *)> &bc_fn);
+
+// ---------------------------------------------------------------------------
 // The rate of change object (FluxLoop + GravitySourceLoop + FluxBC) and the
 // SSP3 time integration with frozen ghost cells.
 class Solver {

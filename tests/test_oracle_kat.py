@@ -138,3 +138,43 @@ def test_rank():
     assert R.rank(A) == 1
     A[0, 1] = 2.5
     assert R.rank(A) == 2
+
+
+def test_ssp3_constant_rhs_is_exact():
+    # test/zisa/unit_test/ode/runge_kutta.cpp:91-110: constant RHS integrated
+    # exactly to 1e-12
+    assert abs(R.rk_ode(2, 0.1, 1.0) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("kind,exact", [(0, np.exp(1.0)), (1, 1.0 / 3.0)])
+def test_ssp3_observed_order(kind, exact):
+    # test/zisa/unit_test/ode/runge_kutta.cpp:172-199: SSP3 order in
+    # [2.9, 3.1] with dt 0.1 -> 0.01 on u' = u and u' = t^2
+    if kind == 1:
+        # SSP3 integrates u' = t^2 exactly (a polynomial of degree 2 in t)
+        assert abs(R.rk_ode(1, 0.1, 1.0) - exact) < 1e-14
+        return
+    e1 = abs(R.rk_ode(kind, 0.1, 1.0) - exact)
+    e2 = abs(R.rk_ode(kind, 0.01, 1.0) - exact)
+    rate = np.log10(e1 / e2)
+    assert 2.9 <= rate <= 3.1, rate
+
+
+def test_two_triangle_grid_reference_kat():
+    # test/zisa/unit_test/grid/grid.cpp:21-107: neighbours, edge numbering
+    # (interior first), the diagonal's normal and the volumes
+    import zisafvm_amd as z
+    from oracle.zfv_ref import Oracle
+
+    verts = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [1.0, 1.0, 0.0], [0.0, 1.0, 0.0]])
+    ids = np.array([[0, 1, 3], [1, 2, 3]], dtype=np.int32)
+    mesh = z.Mesh(2, ids, verts[ids], np.zeros(2, dtype=np.uint8))
+    p = z.make_params(n_dims=2, order=1, n_biased=0)
+    o = Oracle(mesh, p)
+    nbr, eidx, normals = o.topology()
+    assert nbr.tolist() == [[-1, 1, -1], [-1, -1, 0]]
+    assert eidx[0, 1] == 0 and eidx[1, 2] == 0
+    assert (eidx[0, 0], eidx[0, 2], eidx[1, 0], eidx[1, 1]) == (1, 2, 3, 4)
+    assert normals.shape[0] == 5
+    np.testing.assert_allclose(normals[0], [0.5 / np.sqrt(0.5), 0.5 / np.sqrt(0.5), 0.0], atol=1e-12)
+    np.testing.assert_allclose(o.geometry()["volumes"], [0.5, 0.5], atol=1e-12)

@@ -191,6 +191,15 @@ def load_library(path: str = LIB_PATH):
         "zfv_time_kernels": (c_int, [vp, ctypes.c_int32, c_double, ctypes.POINTER(KernelTimes)]),
         "zfv_last_error": (ctypes.c_char_p, []),
         "zfv_bandwidth_probe": (c_int, [ctypes.c_int64, ctypes.c_int32, dp]),
+        "zfv_batch_create": (c_int, [ctypes.POINTER(Params), ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                     ctypes.POINTER(vp)]),
+        "zfv_batch_destroy": (c_int, [vp]),
+        "zfv_batch_add": (c_int, [vp, vp, c_double, c_double, c_double, ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int32)]),
+        "zfv_batch_remove": (c_int, [vp, ctypes.c_int32]),
+        "zfv_batch_run": (c_int, [vp, ctypes.c_int32]),
+        "zfv_batch_query": (c_int, [vp, ctypes.c_int32, dp, dp, ip, ip]),
+        "zfv_batch_set_xcd_chunks": (c_int, [vp, ctypes.c_int32]),
         "zfv_mesh_lattice2d": (
             c_int,
             [ctypes.c_int32, c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -372,3 +381,49 @@ class Solver:
         kt = KernelTimes()
         _check(self.lib.zfv_time_kernels(self.h, n_steps, dt, ctypes.byref(kt)))
         return kt
+
+
+class Batch:
+    """Many instances of one kernel shape advanced together (``zfv_batch``)."""
+
+    def __init__(self, params: Params, cell_capacity: int, face_capacity: int, max_instances: int):
+        self.lib = load_library()
+        b = ctypes.c_void_p()
+        _check(self.lib.zfv_batch_create(ctypes.byref(params), cell_capacity, face_capacity, max_instances,
+                                         ctypes.byref(b)))
+        self.b = b
+        self.members = {}
+
+    def close(self):
+        if self.b:
+            self.lib.zfv_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, solver: Solver, cfl: float, t0: float, t_end: float, max_steps: int = 0) -> int:
+        slot = ctypes.c_int32()
+        _check(self.lib.zfv_batch_add(self.b, solver.h, cfl, t0, t_end, max_steps, ctypes.byref(slot)))
+        self.members[slot.value] = solver
+        return slot.value
+
+    def remove(self, slot: int):
+        _check(self.lib.zfv_batch_remove(self.b, slot))
+        self.members.pop(slot, None)
+
+    def run(self, n_steps: int):
+        _check(self.lib.zfv_batch_run(self.b, n_steps))
+
+    def query(self, slot: int):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        steps, active = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.zfv_batch_query(self.b, slot, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(steps),
+                                        ctypes.byref(active)))
+        return dict(t=t.value, dt=dt.value, steps=steps.value, active=bool(active.value))
+
+    def set_xcd_chunks(self, on: bool):
+        _check(self.lib.zfv_batch_set_xcd_chunks(self.b, int(on)))

@@ -124,6 +124,160 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// ---------------------------------------------------------------------------
+// Batched execution: many instances of one kernel shape advance together.
+// Each instance keeps its own arrays; a device table of DevInst describes
+// them, and block tables map every launched block to (instance, local block).
+// The graph of one step is captured once for the batch's capacity; adding or
+// removing instances only rewrites the device tables between replays.
+
+struct DevInst {
+  Img m;
+  double *U, *X, *K, *fpv, *G, *src;
+  double cfl, t, t_end, dt;
+  unsigned long long dt_bits;
+  int active;
+  int steps_done;
+  int max_steps;
+  int pad;
+};
+
+struct BlockRef {
+  int inst;   // -1: no work
+  int local;  // block index inside the instance
+};
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+__global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
+  const BlockRef b = blk[blockIdx.x];
+  if (b.inst < 0) return;
+  const DevInst &d = insts[b.inst];
+  if (!d.active) return;
+  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
+  if (i >= d.m.n) return;
+  reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(d.m, i, stage == 0 ? d.U : d.X, d.fpv, d.src);
+}
+
+template <int DIM, int Q>
+__global__ __launch_bounds__(256) void kb_face_flux(DevInst *insts, const BlockRef *blk) {
+  const BlockRef b = blk[blockIdx.x];
+  if (b.inst < 0) return;
+  const DevInst &d = insts[b.inst];
+  if (!d.active) return;
+  const int64_t e = int64_t(b.local) * 256 + threadIdx.x;
+  if (e >= d.m.ne) return;
+  face_flux<DIM, Q>(d.m, e, d.fpv, d.G);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void kb_update(DevInst *insts, const BlockRef *blk, int stage) {
+  constexpr int NV = DIM + 2;
+  const BlockRef b = blk[blockIdx.x];
+  if (b.inst < 0) return;
+  const DevInst &d = insts[b.inst];
+  if (!d.active) return;
+  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
+  const Img &m = d.m;
+  const int64_t n = m.n;
+  if (i >= n) return;
+  const double *Us = stage == 0 ? d.U : d.X;
+  double L[NV];
+  cell_rate<DIM>(m, i, Us, d.G, d.src, L);
+  // SSP3 next-state coefficients (src/zisa/ode/runge_kutta.cpp:154-165)
+  const double c0 = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 1.0 / 6);
+  const double c1 = stage == 1 ? 0.25 : 1.0 / 6;
+  const double c2 = 2.0 / 3;
+  const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
+  double *Un = stage == 2 ? d.U : d.X;
+  for (int v = 0; v < NV; ++v) {
+    d.K[(int64_t(stage) * NV + v) * n + i] = L[v];
+    double dudt = 0.0;
+    if (stage == 0) {
+      dudt += c0 * L[v];
+    } else if (stage == 1) {
+      dudt += c0 * d.K[(int64_t(0) * NV + v) * n + i];
+      dudt += c1 * L[v];
+    } else {
+      dudt += c0 * d.K[(int64_t(0) * NV + v) * n + i];
+      dudt += c1 * d.K[(int64_t(1) * NV + v) * n + i];
+      dudt += c2 * L[v];
+    }
+    double un = d.U[v * n + i] + d.dt * dudt;
+    if (ghost) un = m.steady[v * n + i];
+    Un[v * n + i] = un;
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void kb_cfl(DevInst *insts, const BlockRef *blk) {
+  constexpr int NV = DIM + 2;
+  const BlockRef b = blk[blockIdx.x];
+  if (b.inst < 0) return;
+  DevInst &d = insts[b.inst];
+  if (!d.active) return;
+  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
+  const Img &m = d.m;
+  double r = 1.0e300;
+  if (i < m.n) {
+    const int64_t n = m.n;
+    double u[NV];
+    for (int v = 0; v < NV; ++v) u[v] = d.U[v * n + i];
+    double m2 = u[1] * u[1] + u[2] * u[2];
+    if (DIM == 3) m2 += u[3] * u[3];
+    double v2 = m2 / (u[0] * u[0]);
+    double p = (u[NV - 1] - 0.5 * m2 / u[0]) * (m.gamma - 1.0);
+    double a = sqrt(m.gamma * p / u[0]);
+    r = m.inr[i] / (sqrt(v2) + a);
+    if (!(r > 0.0)) r = 0.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) r = fmin(r, __shfl_down(r, off, 64));
+  __shared__ double wmin[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wmin[wid] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bmin = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+    unsigned long long bits;
+    memcpy(&bits, &bmin, sizeof(bits));
+    atomicMin(&d.dt_bits, bits);
+  }
+}
+
+// per-instance step bookkeeping (one thread per instance)
+__global__ void kb_begin_step(DevInst *insts, int n_inst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_inst) return;
+  const double big = 1.0e300;
+  unsigned long long bits;
+  memcpy(&bits, &big, sizeof(bits));
+  insts[k].dt_bits = bits;
+}
+
+__global__ void kb_choose_dt(DevInst *insts, int n_inst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_inst) return;
+  DevInst &d = insts[k];
+  if (!d.active) return;
+  double m;
+  unsigned long long bits = d.dt_bits;
+  memcpy(&m, &bits, sizeof(m));
+  double dt = d.cfl * m;
+  const double rest = d.t_end - d.t;
+  if (dt > rest) dt = rest;
+  d.dt = dt;
+  if (!(dt > 0.0)) d.active = 0;
+}
+
+__global__ void kb_end_step(DevInst *insts, int n_inst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_inst) return;
+  DevInst &d = insts[k];
+  if (!d.active) return;
+  d.t += d.dt;
+  d.steps_done += 1;
+  if (d.t >= d.t_end || d.steps_done >= d.max_steps) d.active = 0;
+}
+
 // plain streaming copy (16-byte vectors, grid-stride) for bandwidth calibration
 __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b,
                                                     int64_t n) {
@@ -140,6 +294,12 @@ struct Launcher {
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
+  // batched variants: grid = number of block-table entries
+  void (*brec)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
+  void (*bface)(DevInst *, const BlockRef *, unsigned, hipStream_t);
+  void (*bupdate)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
+  void (*bcfl)(DevInst *, const BlockRef *, unsigned, hipStream_t);
+  int dim = 2;
 };
 
 inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
@@ -163,6 +323,19 @@ Launcher make_launcher() {
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
     hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
   };
+  l.brec = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, hipStream_t s) {
+    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q>), dim3(nb), dim3(256), 0, s, d, b, stage);
+  };
+  l.bface = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
+    hipLaunchKernelGGL((kb_face_flux<DIM, Q>), dim3(nb), dim3(256), 0, s, d, b);
+  };
+  l.bupdate = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, hipStream_t s) {
+    hipLaunchKernelGGL((kb_update<DIM>), dim3(nb), dim3(256), 0, s, d, b, stage);
+  };
+  l.bcfl = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
+    hipLaunchKernelGGL((kb_cfl<DIM>), dim3(nb), dim3(256), 0, s, d, b);
+  };
+  l.dim = DIM;
   return l;
 }
 
@@ -657,6 +830,223 @@ int zfv_mesh_view(const zfv_mesh_buf *buf, zfv_mesh *view) {
 int zfv_mesh_free(zfv_mesh_buf *buf) {
   delete reinterpret_cast<MeshBuf *>(buf);
   return ZFV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+
+struct zfv_batch {
+  zfv::Launcher L{};
+  zfv::Shape shape{};
+  hipStream_t stream = nullptr;
+  int max_inst = 0;
+  int64_t cap_cell_blocks = 0, cap_face_blocks = 0;
+  bool xcd_chunks = true;
+  zfv::DevInst *d_inst = nullptr;
+  zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr;
+  std::vector<zfv::DevInst> h_inst;
+  std::vector<zfv_handle *> members;
+  hipGraphExec_t graph = nullptr;
+  bool dirty = true;
+  ~zfv_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (d_inst) (void)hipFree(d_inst);
+    if (d_bcell) (void)hipFree(d_bcell);
+    if (d_bface) (void)hipFree(d_bface);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace zfv {
+
+static bool same_shape(const Shape &a, const Shape &b) {
+  return a.dim == b.dim && a.dofc == b.dofc && a.rc == b.rc && a.nb == b.nb && a.rb == b.rb && a.q == b.q;
+}
+
+// Block table: logical blocks of every active instance, in slot order, laid
+// out so that hardware block b (dispatched to XCD b mod 8) serves the
+// (b mod 8)-th contiguous eighth of the logical blocks when xcd_chunks is set:
+// neighbouring cells then share one XCD's L2.
+static std::vector<BlockRef> block_table(const std::vector<int64_t> &counts, int64_t cap, bool xcd_chunks) {
+  std::vector<BlockRef> logical;
+  for (size_t s = 0; s < counts.size(); ++s)
+    for (int64_t k = 0; k < (counts[s] + 255) / 256; ++k) logical.push_back({int(s), int(k)});
+  if (int64_t(logical.size()) > cap) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
+  std::vector<BlockRef> t(cap, BlockRef{-1, 0});
+  if (!xcd_chunks) {
+    std::copy(logical.begin(), logical.end(), t.begin());
+    return t;
+  }
+  const int64_t g8 = cap / 8;
+  for (int64_t j = 0; j < int64_t(logical.size()); ++j) {
+    const int64_t x = j / g8, k = j % g8;
+    t[8 * k + x] = logical[j];
+  }
+  return t;
+}
+
+static void batch_sync_down(zfv_batch *b) {
+  HIP_CHECK(hipMemcpyAsync(b->h_inst.data(), b->d_inst, sizeof(DevInst) * b->max_inst, hipMemcpyDeviceToHost,
+                           b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+}
+
+static void batch_upload(zfv_batch *b) {
+  std::vector<int64_t> cells(b->max_inst, 0), faces(b->max_inst, 0);
+  for (int s = 0; s < b->max_inst; ++s)
+    if (b->members[s] && b->h_inst[s].active) {
+      cells[s] = b->h_inst[s].m.n;
+      faces[s] = b->h_inst[s].m.ne;
+    }
+  auto tc = block_table(cells, b->cap_cell_blocks, b->xcd_chunks);
+  auto tf = block_table(faces, b->cap_face_blocks, b->xcd_chunks);
+  HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst.data(), sizeof(DevInst) * b->max_inst, hipMemcpyHostToDevice,
+                           b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_bcell, tc.data(), sizeof(BlockRef) * tc.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_bface, tf.data(), sizeof(BlockRef) * tf.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->dirty = false;
+}
+
+static void batch_enqueue_step(zfv_batch *b) {
+  const unsigned nc = unsigned(b->cap_cell_blocks), nf = unsigned(b->cap_face_blocks);
+  const unsigned ib = unsigned((b->max_inst + 63) / 64);
+  hipLaunchKernelGGL(kb_begin_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
+  b->L.bcfl(b->d_inst, b->d_bcell, nc, b->stream);
+  hipLaunchKernelGGL(kb_choose_dt, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
+  for (int s = 0; s < 3; ++s) {
+    b->L.brec(b->d_inst, b->d_bcell, nc, s, b->stream);
+    b->L.bface(b->d_inst, b->d_bface, nf, b->stream);
+    b->L.bupdate(b->d_inst, b->d_bcell, nc, s, b->stream);
+  }
+  hipLaunchKernelGGL(kb_end_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
+}
+
+}  // namespace zfv
+
+extern "C" {
+
+int zfv_batch_create(const zfv_params *params, int64_t cell_capacity, int64_t face_capacity, int32_t max_instances,
+                     zfv_batch **out) {
+  ZFV_API_BEGIN
+  if (!params || !out || max_instances < 1 || cell_capacity < 1) throw ZfvError(ZFV_ERR_INVALID, "bad batch arguments");
+  check_device();
+  auto b = std::make_unique<zfv_batch>();
+  b->shape = shape_for(*params);
+  b->L = pick_launcher(b->shape);
+  b->max_inst = max_instances;
+  // capacity in blocks, rounded to a multiple of 8 (one chunk per XCD)
+  auto round8 = [](int64_t x) { return ((x + 7) / 8) * 8; };
+  b->cap_cell_blocks = round8((cell_capacity + 255) / 256 + max_instances);
+  b->cap_face_blocks = round8((std::max<int64_t>(face_capacity, 1) + 255) / 256 + max_instances);
+  HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipMalloc(&b->d_inst, sizeof(DevInst) * max_instances));
+  HIP_CHECK(hipMalloc(&b->d_bcell, sizeof(BlockRef) * b->cap_cell_blocks));
+  HIP_CHECK(hipMalloc(&b->d_bface, sizeof(BlockRef) * b->cap_face_blocks));
+  b->h_inst.assign(max_instances, DevInst{});
+  b->members.assign(max_instances, nullptr);
+  batch_upload(b.get());
+  *out = b.release();
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_destroy(zfv_batch *b) {
+  delete b;
+  return ZFV_OK;
+}
+
+int zfv_batch_set_xcd_chunks(zfv_batch *b, int32_t on) {
+  ZFV_API_BEGIN
+  batch_sync_down(b);
+  b->xcd_chunks = on != 0;
+  batch_upload(b);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_end, int32_t max_steps,
+                  int32_t *slot) {
+  ZFV_API_BEGIN
+  if (!same_shape(b->shape, h->img.s)) throw ZfvError(ZFV_ERR_INVALID, "instance shape differs from the batch's");
+  batch_sync_down(b);
+  int s = -1;
+  for (int k = 0; k < b->max_inst; ++k)
+    if (!b->members[k]) {
+      s = k;
+      break;
+    }
+  if (s < 0) throw ZfvError(ZFV_ERR_MEMORY, "batch is full");
+  DevInst d{};
+  d.m = h->dimg;
+  d.U = h->U;
+  d.X = h->X;
+  d.K = h->K;
+  d.fpv = h->fpv;
+  d.G = h->G;
+  d.src = h->src;
+  d.cfl = cfl;
+  d.t = t0;
+  d.t_end = t_end;
+  d.dt = 0.0;
+  d.active = 1;
+  d.steps_done = 0;
+  d.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
+  b->h_inst[s] = d;
+  b->members[s] = h;
+  try {
+    batch_upload(b);
+  } catch (...) {
+    b->members[s] = nullptr;
+    b->h_inst[s].active = 0;
+    batch_upload(b);
+    throw;
+  }
+  *slot = s;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_remove(zfv_batch *b, int32_t slot) {
+  ZFV_API_BEGIN
+  if (slot < 0 || slot >= b->max_inst || !b->members[slot]) throw ZfvError(ZFV_ERR_INVALID, "bad slot");
+  batch_sync_down(b);
+  b->members[slot] = nullptr;
+  b->h_inst[slot].active = 0;
+  batch_upload(b);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
+  ZFV_API_BEGIN
+  if (b->dirty) batch_upload(b);
+  if (!b->graph) {
+    hipGraph_t g;
+    HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    batch_enqueue_step(b);
+    HIP_CHECK(hipStreamEndCapture(b->stream, &g));
+    HIP_CHECK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(g));
+  }
+  for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
+  batch_sync_down(b);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps_done, int32_t *active) {
+  ZFV_API_BEGIN
+  if (slot < 0 || slot >= b->max_inst || !b->members[slot]) throw ZfvError(ZFV_ERR_INVALID, "bad slot");
+  const DevInst &d = b->h_inst[slot];
+  if (t) *t = d.t;
+  if (dt) *dt = d.dt;
+  if (steps_done) *steps_done = d.steps_done;
+  if (active) *active = d.active;
+  return ZFV_OK;
+  ZFV_API_END
 }
 
 }  // extern "C"

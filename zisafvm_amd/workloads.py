@@ -154,6 +154,26 @@ def isothermal_2d(n=360, order=4, amplitude=1e-6, seed=2, jitter=0.2, ghost_laye
                     meta=dict(n=n, jitter=jitter, seed=seed, amplitude=amplitude, order=order))
 
 
+def batched_2d(n_instances=64, min_cells=5000, max_cells=200000, seed=4, order=3):
+    """Batched workload: instances with triangle counts log-uniform in
+    [min_cells, max_cells]; even members are isentropic vortices (strength
+    U[2, 6]), odd members isothermal atmospheres with a pressure bump of
+    amplitude log-uniform in [1e-7, 1e-3].  All CWENO-AO order 3."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_instances):
+        cells = np.exp(rng.uniform(np.log(min_cells), np.log(max_cells)))
+        n = max(8, int(round(np.sqrt(cells / 2.0))))
+        if k % 2 == 0:
+            wl = vortex_2d(n=n, order=order, beta=float(rng.uniform(2.0, 6.0)), seed=1000 + k)
+        else:
+            amp = float(np.exp(rng.uniform(np.log(1e-7), np.log(1e-3))))
+            wl = isothermal_2d(n=n, order=order, amplitude=amp, seed=1000 + k)
+        wl.meta["member"] = k
+        out.append(wl)
+    return out
+
+
 def polytrope_3d(n=40, order=3, amplitude=1e-5, seed=3, jitter=0.1, radius=1.0, ghost_radius=0.85) -> Workload:
     """Well-balanced 3D: n = 1 polytrope in the unit ball, radial gravity from
     the enclosed mass, Rusanov flux, isentropic equilibrium."""
