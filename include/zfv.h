@@ -184,6 +184,56 @@ int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *
  * cells so stencil gathers share one L2; 0 is the hardware round robin. */
 int zfv_batch_set_xcd_chunks(zfv_batch *b, int32_t on);
 
+/* Device bytes zfv_create would allocate for this mesh (host-only estimate
+ * from the kernel shape and the cell count; used for HBM admission). */
+int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, int64_t *bytes);
+
+/* ---- Serving: continuous batching of solve requests (no counterpart in the
+ * reference).  A request is a mesh, parameters, initial state (and FrozenBC
+ * steady state) and an end time; it runs its own time loop (CFL every step,
+ * clamped to t_end).  Requests are admitted while their estimated device
+ * bytes fit `hbm_budget_bytes`; setup workers build admitted instances while
+ * the GPU keeps stepping; built instances join the batch of their kernel
+ * shape at the next step boundary and leave it when they finish, returning
+ * their memory to the budget. */
+typedef struct zfv_server zfv_server;
+enum {
+  ZFV_REQ_QUEUED = 0,  /* waiting for HBM budget */
+  ZFV_REQ_SETUP = 1,   /* admitted; host setup + upload in progress */
+  ZFV_REQ_RUNNING = 2, /* stepping on the device */
+  ZFV_REQ_DONE = 3,
+  ZFV_REQ_FAILED = 4
+};
+typedef struct zfv_request_status {
+  int32_t state;
+  int32_t steps;        /* SSP3 steps taken */
+  int64_t n_cells;
+  double t_final;       /* simulation time reached */
+  double admit_s;       /* seconds from submit to admission (-1: not yet) */
+  double start_s;       /* seconds from submit to joining a batch */
+  double latency_s;     /* seconds from submit to result (-1: not finished) */
+} zfv_request_status;
+typedef struct zfv_server_stats_t {
+  int64_t reserved_bytes; /* HBM estimate of admitted, unfinished requests */
+  int32_t pending;        /* waiting for HBM */
+  int32_t in_setup;       /* admitted, instance being built or queued for a setup worker */
+  int32_t running;        /* members of a batch */
+  int32_t ready;          /* built, joining at the next step boundary */
+  int64_t cell_stages;  /* cell-stage updates completed so far */
+  int64_t steps_run;    /* step boundaries passed by the GPU loop */
+} zfv_server_stats_t;
+int zfv_server_create(int64_t hbm_budget_bytes, int64_t batch_cell_capacity, int32_t max_members,
+                      int32_t setup_workers, zfv_server **out);
+/* Copies the request; returns at once with its id. */
+int zfv_server_submit(zfv_server *s, const zfv_mesh *mesh, const zfv_params *params, const double *u0,
+                      const double *steady, double t_end, double cfl, int32_t max_steps, int64_t *id);
+int zfv_server_poll(zfv_server *s, int64_t id, zfv_request_status *status);
+/* Final state (n_cells x 5, caller order) of a finished request. */
+int zfv_server_result(zfv_server *s, int64_t id, double *u_out);
+int zfv_server_release(zfv_server *s, int64_t id);
+int zfv_server_stats(zfv_server *s, zfv_server_stats_t *out);
+int zfv_server_destroy(zfv_server *s);
+
 /* Streaming copy of `bytes` bytes between two device buffers, `iters` times,
  * timed with HIP events: the measured HBM copy rate (read + write bytes per
  * second) used to calibrate the roofline and the PMC byte counters.  Has no

@@ -122,9 +122,10 @@ def make_params(
     """Reconstruction parameters in the reference's shape
     (src/zisa/reconstruction/hybrid_weno_params.cpp:26-44: orders {p,2,2,2},
     biases {c,b,b,b}, overfit {2,1.5,1.5,1.5}, linear weights {100,1,1,1},
-    epsilon 1e-6, exponent 4)."""
+    epsilon 1e-6, exponent 4); one one-sided stencil per face by default
+    (three on triangles, four on tetrahedra)."""
     p = Params()
-    nb = (n_dims if n_dims == 3 else 3) if n_biased is None else n_biased
+    nb = (n_dims + 1) if n_biased is None else n_biased
     p.n_dims = n_dims
     p.n_stencils = 1 + nb
     for k in range(5):

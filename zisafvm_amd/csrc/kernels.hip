@@ -412,6 +412,8 @@ static int fail(const std::exception &e) {
   return ZFV_ERR_INVALID;
 }
 
+int api_fail(const std::exception &e) { return fail(e); }
+
 #define ZFV_API_BEGIN try {
 #define ZFV_API_END                     \
   }                                     \
@@ -769,6 +771,22 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
   ZFV_API_END
 }
 
+int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, int64_t *bytes) {
+  ZFV_API_BEGIN
+  const Shape s = shape_for(*params);
+  const double n = mesh->n_cells;
+  const double ne = n * (s.nfaces / 2.0);  // interior faces <= cells * faces / 2
+  const double nv = s.nvars;
+  double per_cell = 8.0 * s.m + 4.0 * s.nrows + 8.0 * (s.nfaces * s.dim + s.dim + 3 + s.nmom) + 4.0 * (2 + s.nfaces) +
+                    8.0 * nv * (1 + 1 + 3);  // U, X, K[3]
+  if (params->gravity != ZFV_GRAVITY_NONE) per_cell += 8.0 * nv;       // source
+  if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
+  double per_face = 8.0 * nv * (2.0 * s.q + 1.0);                      // point values + fluxes
+  *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
 int zfv_bandwidth_probe(int64_t bytes, int32_t iters, double *gbytes_per_s) {
   ZFV_API_BEGIN
   check_device();
@@ -843,6 +861,7 @@ struct zfv_batch {
   hipStream_t stream = nullptr;
   int max_inst = 0;
   int64_t cap_cell_blocks = 0, cap_face_blocks = 0;
+  int64_t n_cell_blocks = 0, n_face_blocks = 0;  // current launch sizes
   bool xcd_chunks = true;
   zfv::DevInst *d_inst = nullptr;
   zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr;
@@ -869,17 +888,20 @@ static bool same_shape(const Shape &a, const Shape &b) {
 // out so that hardware block b (dispatched to XCD b mod 8) serves the
 // (b mod 8)-th contiguous eighth of the logical blocks when xcd_chunks is set:
 // neighbouring cells then share one XCD's L2.
+// The table has as many entries (a multiple of 8) as the members need; the
+// step graph is launched with exactly that many blocks.
 static std::vector<BlockRef> block_table(const std::vector<int64_t> &counts, int64_t cap, bool xcd_chunks) {
   std::vector<BlockRef> logical;
   for (size_t s = 0; s < counts.size(); ++s)
     for (int64_t k = 0; k < (counts[s] + 255) / 256; ++k) logical.push_back({int(s), int(k)});
-  if (int64_t(logical.size()) > cap) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
-  std::vector<BlockRef> t(cap, BlockRef{-1, 0});
+  const int64_t size = std::max<int64_t>(8, ((int64_t(logical.size()) + 7) / 8) * 8);
+  if (size > cap) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
+  std::vector<BlockRef> t(size, BlockRef{-1, 0});
   if (!xcd_chunks) {
     std::copy(logical.begin(), logical.end(), t.begin());
     return t;
   }
-  const int64_t g8 = cap / 8;
+  const int64_t g8 = size / 8;
   for (int64_t j = 0; j < int64_t(logical.size()); ++j) {
     const int64_t x = j / g8, k = j % g8;
     t[8 * k + x] = logical[j];
@@ -902,6 +924,15 @@ static void batch_upload(zfv_batch *b) {
     }
   auto tc = block_table(cells, b->cap_cell_blocks, b->xcd_chunks);
   auto tf = block_table(faces, b->cap_face_blocks, b->xcd_chunks);
+  if (int64_t(tc.size()) != b->n_cell_blocks || int64_t(tf.size()) != b->n_face_blocks) {
+    // launch sizes change: the step graph is captured again on the next run
+    b->n_cell_blocks = int64_t(tc.size());
+    b->n_face_blocks = int64_t(tf.size());
+    if (b->graph) {
+      HIP_CHECK(hipGraphExecDestroy(b->graph));
+      b->graph = nullptr;
+    }
+  }
   HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst.data(), sizeof(DevInst) * b->max_inst, hipMemcpyHostToDevice,
                            b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_bcell, tc.data(), sizeof(BlockRef) * tc.size(), hipMemcpyHostToDevice, b->stream));
@@ -911,7 +942,7 @@ static void batch_upload(zfv_batch *b) {
 }
 
 static void batch_enqueue_step(zfv_batch *b) {
-  const unsigned nc = unsigned(b->cap_cell_blocks), nf = unsigned(b->cap_face_blocks);
+  const unsigned nc = unsigned(b->n_cell_blocks), nf = unsigned(b->n_face_blocks);
   const unsigned ib = unsigned((b->max_inst + 63) / 64);
   hipLaunchKernelGGL(kb_begin_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
   b->L.bcfl(b->d_inst, b->d_bcell, nc, b->stream);
