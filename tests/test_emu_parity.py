@@ -63,3 +63,32 @@ def test_polytrope_3d():
     _stencils_equal(o, e, wl.mesh.n_cells)
     errs = wb_err(e.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, wl.params.gamma)
     assert max(errs) <= TOL, errs
+
+
+def _ssp3(rate, u0, dt, freeze):
+    """The reference's SSP3 in Butcher form (src/zisa/ode/runge_kutta.cpp:122-165)
+    driven by `rate`, with the frozen BC applied after every stage sum."""
+    L0 = rate(u0)
+    u1 = freeze(u0 + dt * L0)
+    L1 = rate(u1)
+    u2 = freeze(u0 + dt * (0.25 * L0 + 0.25 * L1))
+    L2 = rate(u2)
+    return freeze(u0 + dt * (L0 / 6 + L1 / 6 + 2.0 / 3.0 * L2))
+
+
+def test_ghost_cells_evolve_like_the_reference_without_frozen_bc():
+    """Without a frozen BC the ghost cells evolve, so their tendencies must
+    match too: faces between two ghost cells carry no flux
+    (include/zisa/fvm_loops/flux_loop.hpp:71-83) and ghost cells away from
+    the interior use first-order stencils."""
+    mesh = z.ball_mesh_3d(8, radius=1.0, ghost_radius=0.6, jitter=0.1, seed=5)
+    u = W.cell_average(mesh, lambda x: W.conserved(1.0 + 0.2 * x[:, 0], 0.3 + 0.1 * x[:, 1], -0.2, 0.1 * x[:, 2],
+                                                   1.0 + 0.1 * (x ** 2).sum(axis=1), 1.4))
+    p = z.make_params(n_dims=3, order=3, flux=z.ZFV_FLUX_RUSANOV, flux_bc=1)
+    o, e = Oracle(mesh, p), Emu(mesh, p)
+    errs = rel_err(e.rate(u), o.rate(u))  # all cells, ghosts included
+    assert max(errs) <= TOL, errs
+    dt = 0.25 * o.cfl_dt(u, 0.45)
+    ue = _ssp3(e.rate, u, dt, lambda v: v)
+    errs = rel_err((ue - u) / dt, (o.step(u, 0.0, dt) - u) / dt)
+    assert max(errs) <= TOL, errs

@@ -26,8 +26,8 @@ def alone(wl, t_end, max_steps):
 
 @pytest.mark.parametrize("xcd", [True, False])
 def test_batch_members_equal_their_own_time_loops(gpu, xcd):
-    wls = [W.vortex_2d(n=12, beta=4.0), W.isothermal_2d(n=10, amplitude=1e-3), W.vortex_2d(n=20, beta=2.0),
-           W.isothermal_2d(n=16, amplitude=1e-4)]
+    wls = [W.vortex_2d(n=12, beta=4.0), W.isothermal_2d(n=10, amplitude=1e-3, order=3), W.vortex_2d(n=20, beta=2.0),
+           W.isothermal_2d(n=16, amplitude=1e-4, order=3)]
     t_ends = [0.01, 0.02, 0.015, 0.005]
     cap = sum(w.mesh.n_cells for w in wls)
     b = z.Batch(wls[0].params, cell_capacity=cap, face_capacity=2 * cap, max_instances=8)

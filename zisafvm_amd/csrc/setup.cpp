@@ -957,14 +957,22 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   for (int i = 0; i < n; ++i) img.iperm[img.perm[i]] = i;
 
   // faces: interior faces numbered in product order of first encounter; the
-  // left cell is the one with the lower caller index (src/zisa/grid/grid.cpp:415-442)
+  // left cell is the one with the lower caller index (src/zisa/grid/grid.cpp:415-442).
+  // Faces between two ghost cells get no number: the reference's flux loop
+  // drops them (include/zisa/fvm_loops/flux_loop.hpp:71-83; every ghost cell
+  // is an "exterior" cell there, so no such face is ever "interior").
+  constexpr int32_t FID_GHOST_PAIR = -2;
   std::vector<int32_t> fid(size_t(n) * nv, -1);  // caller numbering [i][k]
   int64_t ne = 0;
   for (int ip = 0; ip < n; ++ip) {
     int i = img.perm[ip];
     for (int k = 0; k < nv; ++k) {
       int j = g.nb(i, k);
-      if (j < 0 || fid[size_t(i) * nv + k] >= 0) continue;
+      if (j < 0 || fid[size_t(i) * nv + k] != -1) continue;
+      if (g.ghost[i] && g.ghost[j]) {
+        fid[size_t(i) * nv + k] = FID_GHOST_PAIR;
+        continue;
+      }
       int32_t e = int32_t(ne++);
       fid[size_t(i) * nv + k] = e;
       for (int m = 0; m < nv; ++m)
@@ -1014,7 +1022,8 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
         cf = CFACE_EXTERIOR;
       } else {
         bool right = j < i;  // left = lower caller index
-        cf = (fid[size_t(i) * nv + k] << 2) | (right ? CFACE_SIDE_R : 0);
+        const int32_t e = fid[size_t(i) * nv + k];
+        cf = e == FID_GHOST_PAIR ? CFACE_GHOST_PAIR : (e << 2) | (right ? CFACE_SIDE_R : 0);
         if (right) {
           // express the left cell's vertex order of this face in own corners
           int kj = -1;

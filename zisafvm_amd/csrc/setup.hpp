@@ -37,6 +37,9 @@ enum : int32_t {
 // cface bits: (face_id << 2) | (side << 0) | (exterior << 1)
 constexpr int CFACE_SIDE_R = 1;
 constexpr int CFACE_EXTERIOR = 2;
+// a face between two ghost cells: no flux and no flux BC, geometry only
+// (include/zisa/fvm_loops/flux_loop.hpp:71-83 leaves such faces out)
+constexpr int CFACE_GHOST_PAIR = CFACE_EXTERIOR | CFACE_SIDE_R;
 
 struct HostImage {
   Shape s;

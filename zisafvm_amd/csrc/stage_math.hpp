@@ -816,7 +816,8 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
   for (int f = 0; f < NF; ++f) {
     const int32_t cf = m.cface[f * n + i];
     const bool exterior = (cf & 2) != 0;
-    if (exterior && m.flux_bc == 0) continue;
+    // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC
+    if (exterior && (m.flux_bc == 0 || (cf & 1))) continue;
     double w3[3][3];
     face_vertices<DIM>(fperm, f, vtx, w3);
     double nn[3], t1[3], t2[3], area;
