@@ -150,13 +150,28 @@ int zfv_get_info(zfv_handle *h, zfv_info *info);
 int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_t *l2g_size,
                      int32_t *members, int32_t members_cap);
 
-/* Per-kernel timing of the last zfv_steps call, milliseconds summed over the
- * launches of each kernel (HIP events on the handle's stream). */
+/* Runs n_steps SSP3 steps of fixed dt launch by launch (the state advances)
+ * with HIP events on the handle's stream around every kernel; returns the
+ * milliseconds summed over the launches of each kernel. */
 typedef struct zfv_kernel_times {
   double reconstruct_ms, face_flux_ms, update_ms;
   int32_t n_launches;
 } zfv_kernel_times;
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out);
+
+/* Launch geometry of the reconstruction kernel.  No counterpart in the
+ * reference.  A cell is served by a group of nvars lanes, 64 / nvars cells per
+ * wavefront; a block holds 1, 2 or 4 wavefronts (64, 128, 256 threads).  The
+ * first rate or step call tunes it on the instance's own data by timing every
+ * candidate with HIP events; results never depend on it.  zfv_tune re-tunes
+ * now (rec_threads = 0), fixes the geometry (64, 128 or 256) or only reports
+ * it (-1). */
+typedef struct zfv_tuning {
+  int32_t rec_threads;      /* threads per reconstruction block in use */
+  int32_t cells_per_block;  /* cells per reconstruction block in use */
+  double rec_ms[3];         /* average ms per launch at 64, 128, 256 threads (0: not measured) */
+} zfv_tuning;
+int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out);
 
 /* ---- Batches: many instances of one kernel shape advanced together.
  * Has no counterpart in the reference (which runs one experiment per
@@ -182,6 +197,9 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps);
 int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps_done, int32_t *active);
 /* Block-to-XCD mapping: 1 (default) gives each XCD a contiguous eighth of the
  * cells so stencil gathers share one L2; 0 is the hardware round robin. */
+/* Reconstruction launch geometry of a batch, as zfv_tune; tuned on the
+ * members present at the first zfv_batch_run. */
+int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out);
 int zfv_batch_set_xcd_chunks(zfv_batch *b, int32_t on);
 
 /* Device bytes zfv_create would allocate for this mesh (host-only estimate

@@ -5,7 +5,12 @@
 // (zisafvm_amd/csrc/stage_math.hpp) in host loops, so the CPU test suite can
 // check the product's numerics against the oracle without a GPU.  The
 // product library never loads this; GPU tests call libzfv_hip.so.
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <thread>
 #include <memory>
 #include <string>
 #include <vector>
@@ -25,12 +30,79 @@ struct Emu {
   std::vector<double> U, fpv, G, src;
 };
 
+// The lanes of one cell's group run as NV host threads in lockstep: every
+// bcast is a barrier-separated exchange through shared slots, as the
+// wavefront's cross-lane moves are on the GPU.  A lane that calls bcast a
+// different number of times than its group deadlocks the group; the
+// barrier then times out and the emulation reports the error.
+struct LaneBarrier {
+  int n;
+  std::mutex mu;
+  std::condition_variable cv;
+  int count = 0;
+  long gen = 0;
+  bool broken = false;
+  double slot[8] = {0};
+  explicit LaneBarrier(int n_) : n(n_) {}
+  void wait() {
+    std::unique_lock<std::mutex> lk(mu);
+    if (broken) throw std::runtime_error("lane group broken");
+    const long my = gen;
+    if (++count == n) {
+      count = 0;
+      ++gen;
+      cv.notify_all();
+      return;
+    }
+    if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return gen != my || broken; }) || broken) {
+      broken = true;
+      cv.notify_all();
+      throw std::runtime_error("lane group: bcast calls differ between lanes");
+    }
+  }
+};
+
+struct EmuXch {
+  LaneBarrier *b;
+  int g;
+  double bcast(double v, int k) {
+    b->slot[g] = v;
+    b->wait();
+    const double r = b->slot[k];
+    b->wait();
+    return r;
+  }
+};
+
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
 void run(Emu &e, const double *U, double *L) {
   constexpr int NV = DIM + 2;
   const int64_t n = e.m.n;
-  for (int64_t i = 0; i < n; ++i)
-    reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(e.m, i, U, e.fpv.data(), e.src.empty() ? nullptr : e.src.data());
+  {
+    LaneBarrier bar(NV);
+    std::vector<std::thread> lanes;
+    std::vector<std::string> errs(NV);
+    for (int g = 0; g < NV; ++g)
+      lanes.emplace_back([&, g] {
+        try {
+          EmuXch x{&bar, g};
+          // the physics set the GPU picks for these parameters
+          const bool light = (e.m.eq == 0 || e.m.eq == 2) && (e.m.gravity == 0 || e.m.gravity == 1);
+          double *src = e.src.empty() ? nullptr : e.src.data();
+          for (int64_t i = 0; i < n; ++i) {
+            if (light)
+              reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(e.m, i, g, x, U, e.fpv.data(), src);
+            else
+              reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(e.m, i, g, x, U, e.fpv.data(), src);
+          }
+        } catch (const std::exception &ex) {
+          errs[g] = ex.what();
+        }
+      });
+    for (auto &t : lanes) t.join();
+    for (auto &m : errs)
+      if (!m.empty()) throw ZfvError(ZFV_ERR_INVALID, "emulation: " + m);
+  }
   for (int64_t f = 0; f < e.m.ne; ++f) face_flux<DIM, Q>(e.m, f, e.fpv.data(), e.G.data());
   for (int64_t i = 0; i < n; ++i) {
     double l[NV];
@@ -96,7 +168,8 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.K = p->polytrope_K;
     m.G = p->polytrope_G;
     m.flux_bc = p->flux_bc;
-    std::memset(m.shift, 0, sizeof(m.shift));
+    for (int q = 0; q < 16; ++q)
+      for (int d = 0; d < 3; ++d) m.shift[q][d] = im.shift[3 * q + d];
     const int fdim = im.s.dim == 2 ? 1 : 3;
     for (size_t q = 0; q < im.face_w.size(); ++q) {
       m.fw[q] = im.face_w[q];

@@ -141,3 +141,25 @@ def test_well_balanced_step_with_frozen_ghosts(gpu):
     np.testing.assert_array_equal(ug[ghost][:, [0, 1, 2, 4]], wl.steady[ghost][:, [0, 1, 2, 4]])
     errs = wb_err((ug - wl.u0) / dt, (uo - wl.u0) / dt, wl.mesh, wl.u0, wl.params.gamma)
     assert max(errs) <= TOL
+
+
+def test_launch_geometry_does_not_change_results(gpu):
+    """The autotuner only picks cells per reconstruction block: every
+    geometry gives bit-identical steps."""
+    wl = W.isothermal_2d(n=24, amplitude=1e-3)
+    out = []
+    for threads in (64, 128, 256):
+        s = z.Solver(wl.mesh, wl.params)
+        s.set_state(wl.u0)
+        s.set_steady_state(wl.steady)
+        t = s.tune(threads)
+        assert t.rec_threads == threads and t.cells_per_block == threads // 64 * 16
+        dt = s.cfl_dt(wl.cfl)
+        s.steps(3, 0.0, dt)
+        out.append(s.get_state())
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
+    s = z.Solver(wl.mesh, wl.params)
+    s.set_state(wl.u0)
+    t = s.tune(0)
+    assert t.rec_threads in (64, 128, 256) and all(ms > 0 for ms in t.rec_ms)

@@ -37,12 +37,32 @@ static thread_local std::string g_last_error;
 // ---------------------------------------------------------------------------
 // kernels
 
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+// Cross-lane exchange inside a group of NV consecutive lanes of a wavefront
+// (one cell per group; lane g of the group owns conserved variable g).
+struct WaveGroup {
+  int base;  // first lane of the group
+  __device__ __forceinline__ double bcast(double v, int k) const { return __shfl(v, base + k, 64); }
+};
+
+// groups of NV lanes per 64-wide wavefront (2D: 16 cells, 3D: 12 cells)
+template <int NV>
+constexpr int groups_per_wave() {
+  return 64 / NV;
+}
+
+// k_reconstruct: blocks of `blockDim.x / 64` waves, 64 / NV cells per wave
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 __global__ __launch_bounds__(256) void k_reconstruct(Img m, const double *__restrict__ U, double *__restrict__ fpv,
                                                      double *__restrict__ src) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m.n) return;
-  reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(m, i, U, fpv, src);
+  constexpr int NV = DIM + 2;
+  constexpr int GPW = groups_per_wave<NV>();
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / NV;
+  const int g = lane - grp * NV;
+  const int64_t i = (int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * GPW + grp;
+  if (grp >= GPW || i >= m.n) return;  // whole groups leave together
+  WaveGroup x{lane - g};
+  reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
 }
 
 template <int DIM, int Q>
@@ -147,15 +167,21 @@ struct BlockRef {
   int local;  // block index inside the instance
 };
 
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
+  constexpr int NV = DIM + 2;
+  constexpr int GPW = groups_per_wave<NV>();
   const BlockRef b = blk[blockIdx.x];
   if (b.inst < 0) return;
   const DevInst &d = insts[b.inst];
   if (!d.active) return;
-  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
-  if (i >= d.m.n) return;
-  reconstruct_cell<DIM, DOFC, RC, NB, RB, Q>(d.m, i, stage == 0 ? d.U : d.X, d.fpv, d.src);
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / NV;
+  const int g = lane - grp * NV;
+  const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * GPW + grp;
+  if (grp >= GPW || i >= d.m.n) return;
+  WaveGroup x{lane - g};
+  reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
 }
 
 template <int DIM, int Q>
@@ -289,17 +315,20 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
 // launch plan for one shape
 
 struct Launcher {
-  void (*rec)(const Img &, const double *, double *, double *, hipStream_t);
+  // rec[ph]: physics set ph (PH_LIGHT / PH_FULL); rec_waves: waves per
+  // reconstruction block (1, 2 or 4)
+  void (*rec[2])(const Img &, const double *, double *, double *, int rec_waves, hipStream_t);
   void (*face)(const Img &, const double *, double *, hipStream_t);
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
   // batched variants: grid = number of block-table entries
-  void (*brec)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
+  void (*brec[2])(DevInst *, const BlockRef *, unsigned, int, int rec_waves, hipStream_t);
   void (*bface)(DevInst *, const BlockRef *, unsigned, hipStream_t);
   void (*bupdate)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
   void (*bcfl)(DevInst *, const BlockRef *, unsigned, hipStream_t);
   int dim = 2;
+  int groups_per_wave = 16;  // cells per reconstruction wave
 };
 
 inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
@@ -307,9 +336,15 @@ inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / b
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
 Launcher make_launcher() {
   Launcher l;
-  l.rec = [](const Img &m, const double *U, double *fpv, double *src, hipStream_t s) {
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U,
-                       fpv, src);
+  l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
+    const int cpb = waves * groups_per_wave<DIM + 2>();
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(blocks_for(m.n, cpb)),
+                       dim3(64 * waves), 0, s, m, U, fpv, src);
+  };
+  l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
+    const int cpb = waves * groups_per_wave<DIM + 2>();
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(blocks_for(m.n, cpb)),
+                       dim3(64 * waves), 0, s, m, U, fpv, src);
   };
   l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
     if (m.ne > 0)
@@ -323,8 +358,13 @@ Launcher make_launcher() {
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
     hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
   };
-  l.brec = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, hipStream_t s) {
-    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q>), dim3(nb), dim3(256), 0, s, d, b, stage);
+  l.brec[PH_LIGHT] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int waves, hipStream_t s) {
+    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(nb), dim3(64 * waves), 0, s, d, b,
+                       stage);
+  };
+  l.brec[PH_FULL] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int waves, hipStream_t s) {
+    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(nb), dim3(64 * waves), 0, s, d, b,
+                       stage);
   };
   l.bface = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
     hipLaunchKernelGGL((kb_face_flux<DIM, Q>), dim3(nb), dim3(256), 0, s, d, b);
@@ -336,6 +376,7 @@ Launcher make_launcher() {
     hipLaunchKernelGGL((kb_cfl<DIM>), dim3(nb), dim3(256), 0, s, d, b);
   };
   l.dim = DIM;
+  l.groups_per_wave = groups_per_wave<DIM + 2>();
   return l;
 }
 
@@ -380,6 +421,9 @@ struct zfv_handle {
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = -1.0;
   int graph_steps = 0;
+  int rec_waves = 0;  // waves per reconstruction block; 0 = not tuned yet
+  double rec_ms[3] = {0, 0, 0};
+  int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL)
 
   template <class T>
   T *alloc(size_t count) {
@@ -453,11 +497,56 @@ static StageCoeffs ssp3_coeffs(int stage) {
   return c;
 }
 
+// Autotuner of the reconstruction launch geometry: every candidate (1, 2, 4
+// waves per block) is launched on the instance's own state, one warm-up and
+// five timed launches between HIP events on the handle's stream; the fastest
+// average wins.  The launches only write the face-point and source buffers.
+static const int kRecWaves[3] = {1, 2, 4};
+
+static void tune_rec(zfv_handle *h, const double *Ustate) {
+  const Img &m = h->dimg;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  int best = 4;
+  double best_ms = 1e300;
+  for (int c = 0; c < 3; ++c) {
+    const int w = kRecWaves[c];
+    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, w, h->stream);
+    HIP_CHECK(hipEventRecord(e0, h->stream));
+    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, w, h->stream);
+    HIP_CHECK(hipEventRecord(e1, h->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    h->rec_ms[c] = ms / 5.0;
+    if (h->rec_ms[c] < best_ms) {
+      best_ms = h->rec_ms[c];
+      best = w;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->rec_waves = best;
+}
+
+// the narrowest physics set whose kernel covers this instance
+static int physics_set(const Img &m) {
+  return (m.eq == 0 || m.eq == 2) && (m.gravity == 0 || m.gravity == 1) ? PH_LIGHT : PH_FULL;
+}
+
+// waves per reconstruction block, tuning on first use (never inside a capture)
+static int rec_waves(zfv_handle *h, const double *Ustate) {
+  if (h->rec_waves == 0) tune_rec(h, Ustate);
+  return h->rec_waves;
+}
+
 static void enqueue_step(zfv_handle *h, double dt) {
   const Img &m = h->dimg;
   const double *Us = h->U;
   for (int s = 0; s < 3; ++s) {
-    h->L.rec(m, Us, h->fpv, h->src, h->stream);
+    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_waves, h->stream);
     h->L.face(m, h->fpv, h->G, h->stream);
     double *Un = (s == 2) ? h->U : h->X;
     h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
@@ -539,7 +628,9 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.K = params->polytrope_K;
   m.G = params->polytrope_G;
   m.flux_bc = params->flux_bc;
-  std::memset(m.shift, 0, sizeof(m.shift));
+  for (int q = 0; q < 16; ++q)
+    for (int d = 0; d < 3; ++d) m.shift[q][d] = h->img.shift[3 * q + d];
+  h->ph = physics_set(m);
   if (im.face_w.size() > 8 || im.vol_w.size() > 16) throw ZfvError(ZFV_ERR_INVALID, "quadrature rule too large");
   const int fdim = s.dim == 2 ? 1 : 3;
   for (size_t q = 0; q < im.face_w.size(); ++q) {
@@ -617,7 +708,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
   to_soa(h, u, h->host_stage);
   HIP_CHECK(hipMemcpyAsync(h->X, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
   const Img &m = h->dimg;
-  h->L.rec(m, h->X, h->fpv, h->src, h->stream);
+  h->L.rec[h->ph](m, h->X, h->fpv, h->src, rec_waves(h, h->X), h->stream);
   h->L.face(m, h->fpv, h->G, h->stream);
   StageCoeffs sc{};
   sc.write_rate_only = 1;
@@ -633,6 +724,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
 int zfv_step(zfv_handle *h, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
+  rec_waves(h, h->U);
   enqueue_step(h, dt);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -644,6 +736,7 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
   if (n_steps < 0) throw ZfvError(ZFV_ERR_INVALID, "n_steps < 0");
+  rec_waves(h, h->U);
   if (!h->graph_exec || h->graph_dt != dt) {
     if (h->graph_exec) {
       HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
@@ -715,6 +808,28 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   ZFV_API_END
 }
 
+int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out) {
+  ZFV_API_BEGIN
+  if (rec_threads == 0) {
+    tune_rec(h, h->U);
+  } else if (rec_threads > 0) {
+    if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
+      throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, 64, 128 or 256");
+    h->rec_waves = rec_threads / 64;
+  }
+  if (h->graph_exec && rec_threads >= 0) {  // the captured step holds the old geometry
+    HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
+    h->graph_exec = nullptr;
+  }
+  if (out) {
+    out->rec_threads = 64 * h->rec_waves;
+    out->cells_per_block = h->rec_waves * h->L.groups_per_wave;
+    for (int c = 0; c < 3; ++c) out->rec_ms[c] = h->rec_ms[c];
+  }
+  return ZFV_OK;
+  ZFV_API_END
+}
+
 int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_t *l2g_size, int32_t *members,
                      int32_t members_cap) {
   ZFV_API_BEGIN
@@ -733,6 +848,7 @@ int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out) {
   ZFV_API_BEGIN
   const Img &m = h->dimg;
+  const int waves = rec_waves(h, h->U);
   std::vector<hipEvent_t> ev(2 * 9 * std::max(n_steps, 1));
   for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
   int k = 0;
@@ -740,7 +856,7 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
     const double *Us = h->U;
     for (int s = 0; s < 3; ++s) {
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
-      h->L.rec(m, Us, h->fpv, h->src, h->stream);
+      h->L.rec[h->ph](m, Us, h->fpv, h->src, waves, h->stream);
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       h->L.face(m, h->fpv, h->G, h->stream);
@@ -860,11 +976,14 @@ struct zfv_batch {
   zfv::Shape shape{};
   hipStream_t stream = nullptr;
   int max_inst = 0;
-  int64_t cap_cell_blocks = 0, cap_face_blocks = 0;
-  int64_t n_cell_blocks = 0, n_face_blocks = 0;  // current launch sizes
+  int64_t cap_cell_blocks = 0, cap_face_blocks = 0, cap_rec_blocks = 0;
+  int64_t n_cell_blocks = 0, n_face_blocks = 0, n_rec_blocks = 0;  // current launch sizes
   bool xcd_chunks = true;
+  int rec_waves = 0;  // waves per reconstruction block; 0 = not tuned yet
+  double rec_ms[3] = {0, 0, 0};
+  int ph = 0;         // physics set: the widest any member needs
   zfv::DevInst *d_inst = nullptr;
-  zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr;
+  zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr, *d_brec = nullptr;
   std::vector<zfv::DevInst> h_inst;
   std::vector<zfv_handle *> members;
   hipGraphExec_t graph = nullptr;
@@ -874,6 +993,7 @@ struct zfv_batch {
     if (d_inst) (void)hipFree(d_inst);
     if (d_bcell) (void)hipFree(d_bcell);
     if (d_bface) (void)hipFree(d_bface);
+    if (d_brec) (void)hipFree(d_brec);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -890,10 +1010,11 @@ static bool same_shape(const Shape &a, const Shape &b) {
 // neighbouring cells then share one XCD's L2.
 // The table has as many entries (a multiple of 8) as the members need; the
 // step graph is launched with exactly that many blocks.
-static std::vector<BlockRef> block_table(const std::vector<int64_t> &counts, int64_t cap, bool xcd_chunks) {
+static std::vector<BlockRef> block_table(const std::vector<int64_t> &counts, int64_t per_block, int64_t cap,
+                                         bool xcd_chunks) {
   std::vector<BlockRef> logical;
   for (size_t s = 0; s < counts.size(); ++s)
-    for (int64_t k = 0; k < (counts[s] + 255) / 256; ++k) logical.push_back({int(s), int(k)});
+    for (int64_t k = 0; k < (counts[s] + per_block - 1) / per_block; ++k) logical.push_back({int(s), int(k)});
   const int64_t size = std::max<int64_t>(8, ((int64_t(logical.size()) + 7) / 8) * 8);
   if (size > cap) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
   std::vector<BlockRef> t(size, BlockRef{-1, 0});
@@ -922,12 +1043,24 @@ static void batch_upload(zfv_batch *b) {
       cells[s] = b->h_inst[s].m.n;
       faces[s] = b->h_inst[s].m.ne;
     }
-  auto tc = block_table(cells, b->cap_cell_blocks, b->xcd_chunks);
-  auto tf = block_table(faces, b->cap_face_blocks, b->xcd_chunks);
-  if (int64_t(tc.size()) != b->n_cell_blocks || int64_t(tf.size()) != b->n_face_blocks) {
+  int ph = PH_LIGHT;
+  for (int s = 0; s < b->max_inst; ++s)
+    if (b->members[s] && b->h_inst[s].active) ph = std::max(ph, physics_set(b->h_inst[s].m));
+  if (ph != b->ph && b->graph) {
+    HIP_CHECK(hipGraphExecDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  b->ph = ph;
+  auto tc = block_table(cells, 256, b->cap_cell_blocks, b->xcd_chunks);
+  auto tf = block_table(faces, 256, b->cap_face_blocks, b->xcd_chunks);
+  const int waves = b->rec_waves > 0 ? b->rec_waves : 4;
+  auto tr = block_table(cells, int64_t(waves) * b->L.groups_per_wave, b->cap_rec_blocks, b->xcd_chunks);
+  if (int64_t(tc.size()) != b->n_cell_blocks || int64_t(tf.size()) != b->n_face_blocks ||
+      int64_t(tr.size()) != b->n_rec_blocks) {
     // launch sizes change: the step graph is captured again on the next run
     b->n_cell_blocks = int64_t(tc.size());
     b->n_face_blocks = int64_t(tf.size());
+    b->n_rec_blocks = int64_t(tr.size());
     if (b->graph) {
       HIP_CHECK(hipGraphExecDestroy(b->graph));
       b->graph = nullptr;
@@ -937,18 +1070,58 @@ static void batch_upload(zfv_batch *b) {
                            b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_bcell, tc.data(), sizeof(BlockRef) * tc.size(), hipMemcpyHostToDevice, b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_bface, tf.data(), sizeof(BlockRef) * tf.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_brec, tr.data(), sizeof(BlockRef) * tr.size(), hipMemcpyHostToDevice, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
   b->dirty = false;
 }
 
+// The batch's reconstruction geometry, tuned like a single instance's: each
+// candidate's block table is uploaded and stage-0 reconstructions of all
+// active members are timed (one warm-up, three timed launches).
+static void batch_tune(zfv_batch *b) {
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  int best = 4;
+  double best_ms = 1e300;
+  for (int c = 0; c < 3; ++c) {
+    b->rec_waves = kRecWaves[c];
+    batch_upload(b);
+    const unsigned nr = unsigned(b->n_rec_blocks);
+    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, b->rec_waves, b->stream);
+    HIP_CHECK(hipEventRecord(e0, b->stream));
+    for (int k = 0; k < 3; ++k) b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, b->rec_waves, b->stream);
+    HIP_CHECK(hipEventRecord(e1, b->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    b->rec_ms[c] = ms / 3.0;
+    if (b->rec_ms[c] < best_ms) {
+      best_ms = b->rec_ms[c];
+      best = kRecWaves[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->rec_waves = best;
+  batch_upload(b);
+}
+
+static bool batch_has_active(const zfv_batch *b) {
+  for (int s = 0; s < b->max_inst; ++s)
+    if (b->members[s] && b->h_inst[s].active) return true;
+  return false;
+}
+
 static void batch_enqueue_step(zfv_batch *b) {
-  const unsigned nc = unsigned(b->n_cell_blocks), nf = unsigned(b->n_face_blocks);
+  const unsigned nc = unsigned(b->n_cell_blocks), nf = unsigned(b->n_face_blocks), nr = unsigned(b->n_rec_blocks);
   const unsigned ib = unsigned((b->max_inst + 63) / 64);
   hipLaunchKernelGGL(kb_begin_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
   b->L.bcfl(b->d_inst, b->d_bcell, nc, b->stream);
   hipLaunchKernelGGL(kb_choose_dt, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
   for (int s = 0; s < 3; ++s) {
-    b->L.brec(b->d_inst, b->d_bcell, nc, s, b->stream);
+    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, s, b->rec_waves, b->stream);
     b->L.bface(b->d_inst, b->d_bface, nf, b->stream);
     b->L.bupdate(b->d_inst, b->d_bcell, nc, s, b->stream);
   }
@@ -972,10 +1145,12 @@ int zfv_batch_create(const zfv_params *params, int64_t cell_capacity, int64_t fa
   auto round8 = [](int64_t x) { return ((x + 7) / 8) * 8; };
   b->cap_cell_blocks = round8((cell_capacity + 255) / 256 + max_instances);
   b->cap_face_blocks = round8((std::max<int64_t>(face_capacity, 1) + 255) / 256 + max_instances);
+  b->cap_rec_blocks = round8((cell_capacity + b->L.groups_per_wave - 1) / b->L.groups_per_wave + max_instances);
   HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIP_CHECK(hipMalloc(&b->d_inst, sizeof(DevInst) * max_instances));
   HIP_CHECK(hipMalloc(&b->d_bcell, sizeof(BlockRef) * b->cap_cell_blocks));
   HIP_CHECK(hipMalloc(&b->d_bface, sizeof(BlockRef) * b->cap_face_blocks));
+  HIP_CHECK(hipMalloc(&b->d_brec, sizeof(BlockRef) * b->cap_rec_blocks));
   b->h_inst.assign(max_instances, DevInst{});
   b->members.assign(max_instances, nullptr);
   batch_upload(b.get());
@@ -1054,6 +1229,11 @@ int zfv_batch_remove(zfv_batch *b, int32_t slot) {
 int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   ZFV_API_BEGIN
   if (b->dirty) batch_upload(b);
+  if (b->rec_waves == 0 && batch_has_active(b)) batch_tune(b);
+  if (b->rec_waves == 0) {  // nothing to run on: any geometry will do
+    b->rec_waves = 4;
+    batch_upload(b);
+  }
   if (!b->graph) {
     hipGraph_t g;
     HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1064,6 +1244,30 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   }
   for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
   batch_sync_down(b);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out) {
+  ZFV_API_BEGIN
+  if (b->dirty) batch_upload(b);
+  if (rec_threads == 0) {
+    if (batch_has_active(b)) batch_tune(b);
+  } else if (rec_threads > 0) {
+    if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
+      throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, 64, 128 or 256");
+    b->rec_waves = rec_threads / 64;
+    batch_upload(b);
+  }
+  if (b->graph && rec_threads >= 0) {
+    HIP_CHECK(hipGraphExecDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  if (out) {
+    out->rec_threads = 64 * b->rec_waves;
+    out->cells_per_block = b->rec_waves * b->L.groups_per_wave;
+    for (int c = 0; c < 3; ++c) out->rec_ms[c] = b->rec_ms[c];
+  }
   return ZFV_OK;
   ZFV_API_END
 }

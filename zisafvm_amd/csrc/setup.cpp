@@ -865,6 +865,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   std::vector<double> Pc(size_t(n) * s.m, 0.0);
   std::vector<int32_t> rows_c(size_t(n) * s.nrows, 0);
   std::vector<double> cond(n, 0.0);
+  std::vector<V3> shift_table(1, V3{0.0, 0.0, 0.0});
   int err = 0;
   std::string err_msg;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -896,6 +897,29 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
           for (int r = 0; r < rows; ++r) P[off + a * rmax + r] = Pk[size_t(a) * rows + r];
         for (int r = 0; r < rows; ++r) R[roff + r] = c.st[k][r + 1].cell;
       }
+      // periodic translation codes of the members (read by the equilibrium
+      // averages of well-balanced runs)
+      for (int k = 0; k < c.n; ++k) {
+        int rows = c.size[k] - 1;
+        int roff = (k == 0 ? 0 : s.rc + (k - 1) * s.rb);
+        if (c.order[k] <= 1 || rows <= 0) continue;
+        for (int r = 0; r < rows; ++r) {
+          const V3 &t = c.st[k][r + 1].shift;
+          if (t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0) continue;
+          int code = -1;
+#pragma omp critical(zfv_shift_table)
+          {
+            for (int q = 1; q < int(shift_table.size()); ++q)
+              if (shift_table[q][0] == t[0] && shift_table[q][1] == t[1] && shift_table[q][2] == t[2]) code = q;
+            if (code < 0 && shift_table.size() < 16) {
+              shift_table.push_back(t);
+              code = int(shift_table.size()) - 1;
+            }
+          }
+          if (code < 0) throw ZfvError(ZFV_ERR_INVALID, "more than 15 distinct periodic translations in the stencils");
+          R[roff + r] |= int32_t(code) << 27;
+        }
+      }
     } catch (const std::exception &e) {
 #pragma omp critical
       {
@@ -906,6 +930,9 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   }
   if (err) throw ZfvError(ZFV_ERR_INVALID, err_msg);
   img.max_cond = *std::max_element(cond.begin(), cond.end());
+  if (n > (1 << 27)) throw ZfvError(ZFV_ERR_INVALID, "more than 2^27 cells");
+  for (size_t q = 0; q < shift_table.size(); ++q)
+    for (int d = 0; d < 3; ++d) img.shift[3 * q + d] = shift_table[q][d];
 
   // reference-layout stencil export (caller numbering)
   img.l2g_off.assign(n + 1, 0);
@@ -999,7 +1026,10 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   for (int ip = 0; ip < n; ++ip) {
     const int i = img.perm[ip];
     for (int m = 0; m < s.m; ++m) img.P[size_t(m) * N + ip] = Pc[size_t(i) * s.m + m];
-    for (int r = 0; r < s.nrows; ++r) img.rows[size_t(r) * N + ip] = img.iperm[rows_c[size_t(i) * s.nrows + r]];
+    for (int r = 0; r < s.nrows; ++r) {
+      const int32_t rc = rows_c[size_t(i) * s.nrows + r];
+      img.rows[size_t(r) * N + ip] = img.iperm[rc & ((1 << 27) - 1)] | (rc & ~((1 << 27) - 1));
+    }
     for (int k = 0; k < nv; ++k)
       for (int d = 0; d < dim; ++d) img.crn[size_t(k * dim + d) * N + ip] = g.c(i, k)[d];
     for (int d = 0; d < dim; ++d) img.xc[size_t(d) * N + ip] = g.ctr[i][d];

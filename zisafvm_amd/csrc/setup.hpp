@@ -63,6 +63,9 @@ struct HostImage {
   // normalized quadrature rules used by the kernels (device constants)
   std::vector<double> face_w, face_x;  // q weights; face_x: q * (dim) barycentric / GL coordinates
   std::vector<double> vol_w, vol_x;    // qv weights; vol_x: qv * (dim+1) barycentric
+  // periodic translations of stencil members: row code c (bits 27..30 of a
+  // row entry) adds shift[3c..3c+2] to the member's coordinates; code 0 = none
+  std::vector<double> shift = std::vector<double>(48, 0.0);
   double max_cond = 0.0;               // largest estimated condition number of A^T A
   double linear_weights[ZFV_MAX_STENCILS] = {0};
 };

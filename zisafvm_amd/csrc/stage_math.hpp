@@ -76,13 +76,22 @@ ZFV_HD double dmin(double a, double b) { return a < b ? a : b; }
 ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
 
 // ---- gravity -----------------------------------------------------------------
-template <int DIM>
+// PH selects the physics a kernel is compiled for: PH_LIGHT covers no or axial
+// gravity with no or isothermal equilibrium (exp only); PH_FULL adds radial
+// and polytrope gravity and the isentropic equilibrium.  The transcendental
+// functions of the full set need many registers, so runs that do not need
+// them use kernels without them.
+constexpr int PH_LIGHT = 0;
+constexpr int PH_FULL = 1;
+
+template <int DIM, int PH = PH_FULL>
 ZFV_HD double phi(const Img &m, const double *x) {
   if (m.gravity == 1) {
     double d = x[0] * m.axis[0] + x[1] * m.axis[1];
     if (DIM == 3) d += x[2] * m.axis[2];
     return m.g * d;
   }
+  if (PH == PH_LIGHT) return 0.0;
   double r2 = x[0] * x[0] + x[1] * x[1];
   if (DIM == 3) r2 += x[2] * x[2];
   double r = sqrt(r2);
@@ -95,10 +104,10 @@ ZFV_HD double phi(const Img &m, const double *x) {
   return 0.0;
 }
 
-template <int DIM>
+template <int DIM, int PH = PH_FULL>
 ZFV_HD void grad_phi(const Img &m, const double *x, double *gp) {
-  if (m.gravity == 1) {
-    for (int d = 0; d < DIM; ++d) gp[d] = m.g * m.axis[d];
+  if (m.gravity == 1 || PH == PH_LIGHT) {
+    for (int d = 0; d < DIM; ++d) gp[d] = m.gravity == 1 ? m.g * m.axis[d] : 0.0;
     return;
   }
   double r2 = x[0] * x[0] + x[1] * x[1];
@@ -117,11 +126,11 @@ ZFV_HD void grad_phi(const Img &m, const double *x, double *gp) {
 }
 
 // ---- local equilibrium: theta -> (rho, E) at x ------------------------------
-template <int DIM>
+template <int DIM, int PH = PH_FULL>
 ZFV_HD void eq_eval(const Img &m, const double *th, double phi_ref, const double *x, double &rho, double &E) {
   const double gamma = m.gamma;
-  if (m.eq == 1) {
-    double h = th[0] + phi_ref - phi<DIM>(m, x);
+  if (PH == PH_FULL && m.eq == 1) {
+    double h = th[0] + phi_ref - phi<DIM, PH>(m, x);
     double K = th[1];
     double base = 1.0 / K * (gamma - 1.0) / gamma * h;
     rho = pow(base, 1.0 / (gamma - 1.0));
@@ -129,7 +138,7 @@ ZFV_HD void eq_eval(const Img &m, const double *th, double phi_ref, const double
     E = p / (gamma - 1.0);
   } else {
     double RT = th[1];
-    rho = th[0] * exp(-(phi<DIM>(m, x) - phi_ref) / RT);
+    rho = th[0] * exp(-(phi<DIM, PH>(m, x) - phi_ref) / RT);
     E = rho * RT / (gamma - 1.0);
   }
 }
@@ -773,6 +782,420 @@ ZFV_HD void reconstruct_cell(const Img &m, int64_t i, const double *__restrict__
       double sum = wb ? (s_acc[v] + 1.0 * cs_[v]) : 1.0 * cs_[v];
       src[v * n + i] = sum / vol_i;
     }
+  }
+}
+
+// ---- the per-cell reconstruction, one lane per conserved variable --------------
+//
+// The same computation as reconstruct_cell, spread over a group of NV lanes
+// that share one cell: lane g owns variable g (rho, momentum components,
+// energy).  Each lane keeps only its variable's polynomial coefficients, so a
+// lane needs a fraction of the registers of a whole cell.  The lanes exchange
+// values through X::bcast(v, k) (value v of lane k of the same group):
+//   - the equilibrium averages of the stencil members are computed by lane
+//     (row mod NV) and broadcast to the density and energy lanes,
+//   - the smoothness indicator of a stencil is the maximum over the variables,
+//   - the face-frame rotation and the gravity source mix momentum components.
+// Every call of bcast sits in control flow that is uniform over the group.
+// Arithmetic per variable follows reconstruct_cell operation for operation.
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, class X>
+ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const double *__restrict__ U,
+                             double *__restrict__ fpv, double *__restrict__ src) {
+  constexpr int NV = DIM + 2;
+  constexpr int NF = DIM + 1;
+  constexpr int NC = DOFC - 1;
+  constexpr int NBC = DIM;
+  constexpr int DEGC = deg_of<DIM>(DOFC);
+  constexpr int NR = RC + NB * RB;
+  const int64_t n = m.n;
+  const double gamma = m.gamma;
+
+  // own state and scale (include/zisa/model/characteristic_scale.hpp:23-32)
+  double u0[NV];
+  for (int v = 0; v < NV; ++v) u0[v] = U[v * n + i];
+  double msq = u0[1] * u0[1] + u0[2] * u0[2];
+  if (DIM == 3) msq += u0[3] * u0[3];
+  const double rho0 = u0[0];
+  const double Eint0 = u0[NV - 1] - 0.5 * msq / rho0;
+  const double cs = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
+  const double scale_g = g == 0 ? rho0 : (g == NV - 1 ? Eint0 : cs);
+  const double inv_g = 1.0 / scale_g;
+  const bool is_rho = g == 0, is_E = g == NV - 1;
+
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+
+  // local equilibrium of this cell: every lane solves it (identical results)
+  double th[2] = {0.0, 0.0};
+  double phi_ref = 0.0;
+  const bool wb = m.eq != 0;
+  bool eq_found = false;
+  if (wb) {
+    double xref[3];
+    volume_point<DIM>(m, 0, vtx, xref);
+    phi_ref = phi<DIM, PH>(m, xref);
+    double vol_i = m.vol[i];
+    auto cell_avg = [&](const double *t, double &ar, double &aE) {
+      double sr = 0.0, sE = 0.0;
+      for (int q = 0; q < m.qv; ++q) {
+        double x[3];
+        volume_point<DIM>(m, q, vtx, x);
+        double r, E;
+        eq_eval<DIM, PH>(m, t, phi_ref, x, r, E);
+        double w = vol_i * m.vw[q];
+        if (q == 0) {
+          sr = w * r;
+          sE = w * E;
+        } else {
+          sr = sr + w * r;
+          sE = sE + w * E;
+        }
+      }
+      ar = (1.0 * sr) / vol_i;
+      aE = (1.0 * sE) / vol_i;
+    };
+    if (PH == PH_LIGHT || m.eq == 2) {
+      double RT = (gamma - 1.0) * Eint0 / rho0;
+      double t1[2] = {1.0, RT};
+      double ar, aE;
+      cell_avg(t1, ar, aE);
+      th[0] = rho0 / ar;
+      th[1] = RT;
+      eq_found = true;
+    } else {
+      // quasi-Newton with a central finite-difference Jacobian
+      // (include/zisa/math/quasi_newton.hpp:12-50)
+      double p = Eint0 * (gamma - 1.0);
+      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / pow(rho0, gamma)};
+      double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
+      double x0 = guess[0], x1 = guess[1];
+      auto f = [&](double a, double b, double &f0, double &f1) {
+        double t[2] = {a, b};
+        double ar, aE;
+        cell_avg(t, ar, aE);
+        f0 = rho0 - ar;
+        f1 = Eint0 - aE;
+      };
+      double fx0, fx1;
+      f(x0, x1, fx0, fx1);
+      double dx0 = 2.0 * atol[0] + 1.0, dx1 = 2.0 * atol[1] + 1.0;
+      double h00 = 0, h01 = 0, h10 = 0, h11 = 0;  // previous two steps
+      bool ok = true;
+      int iter = 0;
+      while (!(fabs(dx0) <= atol[0] && fabs(dx1) <= atol[1]) && iter < 20) {
+        double J00, J01, J10, J11;
+        {
+          double e = 1e-6 * fabs(x0);
+          double fp0, fp1, fm0, fm1;
+          f(x0 + 0.5 * e, x1, fp0, fp1);
+          f(x0 - 0.5 * e, x1, fm0, fm1);
+          J00 = (fp0 - fm0) / e;
+          J01 = (fp1 - fm1) / e;
+        }
+        {
+          double e = 1e-6 * fabs(x1);
+          double fp0, fp1, fm0, fm1;
+          f(x0, x1 + 0.5 * e, fp0, fp1);
+          f(x0, x1 - 0.5 * e, fm0, fm1);
+          J10 = (fp0 - fm0) / e;
+          J11 = (fp1 - fm1) / e;
+        }
+        double inv_det = 1.0 / (J00 * J11 - J01 * J10);
+        dx0 = inv_det * (J11 * fx0 - J10 * fx1);
+        dx1 = inv_det * (-J01 * fx0 + J00 * fx1);
+        x0 -= dx0;
+        x1 -= dx1;
+        f(x0, x1, fx0, fx1);
+        if (iter >= 4 && !(dx0 <= atol[0] && dx1 <= atol[1])) {
+          double rate0 = log(fabs(dx0) / fabs(h10)) / log(fabs(h10) / fabs(h00));
+          double rate1 = log(fabs(dx1) / fabs(h11)) / log(fabs(h11) / fabs(h01));
+          if (!(rate0 >= 0.0) || !(rate1 >= 0.0)) ok = false;
+        }
+        h00 = h10;
+        h01 = h11;
+        h10 = dx0;
+        h11 = dx1;
+        if (!ok) break;
+        ++iter;
+      }
+      if (ok && !(iter == 20 && !(fabs(dx0) <= 1000.0 * atol[0] && fabs(dx1) <= 1000.0 * atol[1]))) {
+        th[0] = x0;
+        th[1] = x1;
+        eq_found = true;
+      }
+    }
+  }
+
+  // equilibrium average of this cell's equilibrium over cell j (shifted)
+  auto member_eq = [&](int64_t j, int code, double &er, double &eE) {
+    double w3[DIM + 1][3];
+    load_corners<DIM>(m, j, w3);
+    for (int k = 0; k < DIM + 1; ++k)
+      for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
+    double vol_j = m.vol[j];
+    double sr = 0.0, sE = 0.0;
+    for (int q = 0; q < m.qv; ++q) {
+      double x[3];
+      volume_point<DIM>(m, q, w3, x);
+      double r, E;
+      eq_eval<DIM, PH>(m, th, phi_ref, x, r, E);
+      double w = vol_j * m.vw[q];
+      if (q == 0) {
+        sr = w * r;
+        sE = w * E;
+      } else {
+        sr = sr + w * r;
+        sE = sE + w * E;
+      }
+    }
+    er = (1.0 * sr) / vol_j;
+    eE = (1.0 * sE) / vol_j;
+  };
+  const bool eq_sub = wb && eq_found;
+
+  double q0;
+  {
+    double er = 0.0, eE = 0.0;
+    if (eq_sub) member_eq(i, 0, er, eE);
+    q0 = u0[g];
+    if (is_rho) q0 -= er;
+    if (is_E) q0 -= eE;
+    q0 = q0 * inv_g;
+  }
+
+  const int32_t meta = m.meta[i];
+  const bool first_order = (meta & 8) != 0;
+  const int kh = meta & 7;
+
+  // this variable's coefficients: central c[alpha] (alpha = 1..DOFC-1),
+  // one-sided b[k][alpha] (alpha = 1..DIM)
+  double c[NC];
+  double b[NB > 0 ? NB : 1][NBC];
+  for (int a = 0; a < NC; ++a) c[a] = 0.0;
+  for (int k = 0; k < NB; ++k)
+    for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
+
+  if (!first_order) {
+    // rows (central, then one-sided) in chunks of NV: lane k of the group
+    // computes the equilibrium average of row r0 + k, then every lane
+    // accumulates the chunk's rows.  The chunk loop stays rolled so that only
+    // one chunk's loads are in flight; the one-sided target stencil is picked
+    // with compile-time indices so b stays in registers.
+#pragma unroll 1
+    for (int r0 = 0; r0 < NR; r0 += NV) {
+      double my_er = 0.0, my_eE = 0.0;
+      if (eq_sub && r0 + g < NR) {
+        const int32_t rw = m.rows[(r0 + g) * n + i];
+        member_eq(rw & ROW_CELL_MASK, rw >> ROW_CELL_BITS, my_er, my_eE);
+      }
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        double er = 0.0, eE = 0.0;
+        if (wb) {
+          er = xch.bcast(my_er, k);
+          eE = xch.bcast(my_eE, k);
+        }
+        const int r = r0 + k;
+        if (r < NR) {
+          const int32_t rw = m.rows[r * n + i];
+          const int64_t j = rw & ROW_CELL_MASK;
+          double d = U[g * n + j];
+          if (wb) {
+            if (is_rho) d -= er;
+            if (is_E) d -= eE;
+          }
+          d = d * inv_g - q0;
+          if (r < RC) {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) c[a] += m.P[(a * RC + r) * n + i] * d;
+          } else {
+            const int kb = (r - RC) / RB;
+            const int rr = (r - RC) - kb * RB;
+            const int off = NC * RC + kb * NBC * RB;
+#pragma unroll
+            for (int a = 0; a < NBC; ++a) {
+              const double p = m.P[(off + a * RB + rr) * n + i];
+#pragma unroll
+              for (int kk = 0; kk < NB; ++kk)
+                if (kk == kb) b[kk][a] += p * d;
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // final polynomial of this variable
+  double a[DOFC];
+  if (first_order) {
+    a[0] = 0.0 + 1.0 * q0;
+    for (int al = 1; al < DOFC; ++al) a[al] = 0.0;
+  } else {
+    double a0[NB + 1];
+    for (int k = 0; k <= NB; ++k) a0[k] = q0;
+    if (m.mode == 0) {
+      // CWENO: p_hi <- (p_hi - sum_{k != hi} c_k p_k) * (1 / c_hi)
+      if (kh == 0) {
+#pragma unroll
+        for (int k = 1; k <= NB; ++k) {
+          const double w = m.lw[k];
+          a0[0] = a0[0] - w * a0[k];
+          for (int al = 0; al < NBC; ++al) c[al] = c[al] - w * b[k - 1][al];
+        }
+        const double iw = 1.0 / m.lw[0];
+        a0[0] = iw * a0[0];
+        for (int al = 0; al < NC; ++al) c[al] = iw * c[al];
+      } else if (NB > 0) {
+        // the corrected polynomial is a one-sided one (central stencil reduced
+        // to order 1); compile-time indices keep b in registers
+#pragma unroll
+        for (int hb = 0; hb < NB; ++hb) {
+          if (hb != kh - 1) continue;
+#pragma unroll
+          for (int k = 0; k <= NB; ++k) {
+            if (k == hb + 1) continue;
+            const double w = m.lw[k];
+            a0[hb + 1] = a0[hb + 1] - w * a0[k];
+            for (int al = 0; al < NBC; ++al) {
+              double o = (k == 0) ? c[al] : b[k > 0 ? k - 1 : 0][al];
+              b[hb][al] = b[hb][al] - w * o;
+            }
+          }
+          const double iw = 1.0 / m.lw[hb + 1];
+          a0[hb + 1] = iw * a0[hb + 1];
+          for (int al = 0; al < NBC; ++al) b[hb][al] = iw * b[hb][al];
+        }
+      }
+    }
+    // smoothness indicators (maximum over the variables, in variable order)
+    // and nonlinear weights (src/zisa/reconstruction/hybrid_weno.cpp:110-127)
+    double alpha[NB + 1];
+    double al_tot = 0.0;
+#pragma unroll
+    for (int k = 0; k <= NB; ++k) {
+      double s = 0.0;
+      if (k == 0) {
+        for (int al = 0; al < NC; ++al) s += c[al] * c[al];
+      } else {
+        for (int al = 0; al < NBC; ++al) s += b[k - 1][al] * b[k - 1][al];
+      }
+      double IS = xch.bcast(s, 0);
+#pragma unroll
+      for (int v = 1; v < NV; ++v) IS = dmax(IS, xch.bcast(s, v));
+      double isp = m.exponent == 4.0 ? sqr(sqr(IS)) : pow(IS, m.exponent);
+      alpha[k] = m.lw[k] / (m.eps + isp);
+      al_tot += alpha[k];
+    }
+    const double w0 = alpha[0] / al_tot;
+    a[0] = 0.0 + w0 * a0[0];
+    for (int al = 1; al < DOFC; ++al) a[al] = 0.0 + w0 * c[al - 1];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const double w = alpha[k + 1] / al_tot;
+      a[0] = a[0] + w * a0[k + 1];
+      for (int al = 1; al <= NBC; ++al) a[al] = a[al] + w * b[k][al - 1];
+    }
+  }
+
+  // moments: zero for alpha < 3
+  double cm[DOFC];
+  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
+  double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
+  const double len = m.len[i];
+  const uint32_t fperm = m.fperm[i];
+
+  // this variable of the scaled reconstruction (+ equilibrium background) at
+  // x; the equilibrium point values come back in r_eq, E_eq
+  auto eval = [&](const double *x, double &u, double &delta, double &r_eq, double &E_eq) {
+    double Xs[3] = {(x[0] - xc[0]) / len, (x[1] - xc[1]) / len, DIM == 3 ? (x[2] - xc[2]) / len : 0.0};
+    double px;
+    poly_eval<DIM, DEGC, 1>(reinterpret_cast<const double(*)[1]>(a), cm, Xs, &px);
+    delta = scale_g * px;
+    u = delta;
+    r_eq = 0.0;
+    E_eq = 0.0;
+    if (eq_sub) {
+      eq_eval<DIM, PH>(m, th, phi_ref, x, r_eq, E_eq);
+      u = is_rho ? r_eq + delta : (is_E ? E_eq + delta : 0.0 + delta);
+    }
+  };
+
+  double s_acc = 0.0;
+  const bool gravity_src = (m.gravity != 0) && (src != nullptr);
+
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int32_t cf = m.cface[f * n + i];
+    double w3[3][3];
+    face_vertices<DIM>(fperm, f, vtx, w3);
+    double nn[3], t1[3], t2[3], area;
+    face_frame<DIM>(w3, nn, t1, t2, area);
+    const bool exterior = (cf & 2) != 0;
+    // outward normal sign for the source term (src/zisa/math/face.cpp:24-27)
+    double x0f[3];
+    face_point<DIM>(m, 0, w3, x0f);
+    double sgn = 0.0;
+    if (gravity_src && wb) {
+      double dd = nn[0] * (x0f[0] - xc[0]) + nn[1] * (x0f[1] - xc[1]) + (DIM == 3 ? nn[2] * (x0f[2] - xc[2]) : 0.0);
+      sgn = double((0.0 < dd) - (dd < 0.0));
+    }
+    double fs = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double x[3];
+      face_point<DIM>(m, q, w3, x);
+      double r_eq = 0.0, E_eq = 0.0;
+      if (!exterior) {
+        double u, dl;
+        eval(x, u, dl, r_eq, E_eq);
+        // rotate into the face frame: momentum lanes need every component
+        double mc[3];
+        for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
+        double r = u;
+        if (g == 1) r = DIM == 2 ? mc[0] * nn[0] + mc[1] * nn[1] : mc[0] * nn[0] + mc[1] * nn[1] + mc[2] * nn[2];
+        if (g == 2) r = DIM == 2 ? mc[0] * t1[0] + mc[1] * t1[1] : mc[0] * t1[0] + mc[1] * t1[1] + mc[2] * t1[2];
+        if (DIM == 3 && g == 3) r = mc[0] * t2[0] + mc[1] * t2[1] + mc[2] * t2[2];
+        const int64_t e = cf >> 2;
+        const int side = cf & 1;
+        fpv[((side * Q + q) * NV + g) * m.ne + e] = r;
+      } else if (gravity_src && eq_sub) {
+        eq_eval<DIM, PH>(m, th, phi_ref, x, r_eq, E_eq);
+      }
+      if (gravity_src && eq_sub && g >= 1 && g <= DIM) {
+        double p_eq = E_eq * (gamma - 1.0);
+        double w = area * m.fw[q];
+        double t = p_eq * (sgn * nn[g - 1]);
+        fs = (q == 0) ? w * t : fs + w * t;
+      }
+    }
+    if (gravity_src && wb) s_acc += 1.0 * fs;
+  }
+
+  if (gravity_src) {
+    const double vol_i = m.vol[i];
+    double cs_ = 0.0;
+    for (int q = 0; q < m.qv; ++q) {
+      double x[3];
+      volume_point<DIM>(m, q, vtx, x);
+      double u, dl, r_eq, E_eq;
+      eval(x, u, dl, r_eq, E_eq);
+      double gp[3];
+      grad_phi<DIM, PH>(m, x, gp);
+      const double drho = xch.bcast(wb ? dl : u, 0);
+      double mc[3];
+      for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
+      double sv = 0.0;
+      if (g >= 1 && g <= DIM) sv = -drho * gp[g - 1];
+      if (is_E) {
+        double mg = 0.0;
+        for (int d = 0; d < DIM; ++d) mg += mc[d] * gp[d];
+        sv = -mg;
+      }
+      double w = vol_i * m.vw[q];
+      cs_ = (q == 0) ? w * sv : cs_ + w * sv;
+    }
+    double sum = wb ? (s_acc + 1.0 * cs_) : 1.0 * cs_;
+    src[g * n + i] = sum / vol_i;
   }
 }
 
