@@ -155,6 +155,37 @@ def batched_vs_plain(z, steps):
     }
 
 
+def serving(z, rates, n_requests, budget_gib):
+    """Poisson streams of mixed 2D/3D solve requests through the serving
+    runtime (continuous batching, HBM-budget admission); one pass per rate."""
+    from zisafvm_amd import serving as S
+    from zisafvm_amd import workloads as W
+
+    t0 = time.perf_counter()
+    reqs = W.serving_requests(n_requests)
+    t_gen = time.perf_counter() - t0
+    budget = int(budget_gib * (1 << 30))
+    est = [S.estimate_device_bytes(w.mesh, w.params) for w in reqs]
+    runs = []
+    for rate in rates:
+        srv = S.Server(budget, batch_cell_capacity=1 << 23, max_members=64, setup_workers=2)
+        res = S.poisson_run(srv, reqs, rate, seed=int(rate * 7))
+        srv.close()
+        runs.append(res.summary())
+        log(f"serving rate {rate}/s: {json.dumps(runs[-1])}")
+    return {
+        "requests": n_requests,
+        "mix": "every 4th a 3D polytrope (10k-60k tets), others 2D vortex / isothermal (10k-200k triangles), "
+               "t_end U[0.05, 0.5], CWENO-AO 3",
+        "cells": [int(w.mesh.n_cells) for w in reqs],
+        "hbm_budget_bytes": budget,
+        "max_request_estimate_bytes": int(max(est)),
+        "request_generation_s": t_gen,
+        "latency_definition": "submit to result, including host setup of the instance",
+        "runs": runs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +197,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates (requests/s)")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-budget-gib", type=float, default=1.0)
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the streaming-copy calibration kernel (1 GiB x 5) and exit")
     args = ap.parse_args()
@@ -242,6 +277,10 @@ def main():
         log("batched vs plain ...")
         out["batched_vs_plain"] = batched_vs_plain(z, args.batch_steps)
         log(json.dumps(out["batched_vs_plain"]))
+    if not args.no_serve:
+        log("serving ...")
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        out["serving"] = serving(z, rates, args.serve_requests, args.serve_budget_gib)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)

@@ -163,3 +163,16 @@ def test_launch_geometry_does_not_change_results(gpu):
     s.set_state(wl.u0)
     t = s.tune(0)
     assert t.rec_threads in (64, 128, 256) and all(ms > 0 for ms in t.rec_ms)
+
+
+def test_gpu_rounds_like_the_host_emulation(gpu):
+    """No multiply-add is fused behind the source's back (-ffp-contract=off,
+    explicit fma), and a plain run uses only correctly rounded operations
+    (+ - * / sqrt): the device then reproduces the host emulation of the
+    same per-cell code bit for bit."""
+    from emu_binding import Emu
+
+    wl = W.vortex_2d(n=16)
+    s = z.Solver(wl.mesh, wl.params)
+    e = Emu(wl.mesh, wl.params)
+    np.testing.assert_array_equal(s.rate(wl.u0), e.rate(wl.u0))

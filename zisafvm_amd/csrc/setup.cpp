@@ -865,7 +865,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   std::vector<double> Pc(size_t(n) * s.m, 0.0);
   std::vector<int32_t> rows_c(size_t(n) * s.nrows, 0);
   std::vector<double> cond(n, 0.0);
-  std::vector<V3> shift_table(1, V3{0.0, 0.0, 0.0});
+  std::vector<V3> row_shift(p.equilibrium == ZFV_EQ_NONE ? 0 : size_t(n) * s.nrows, V3{0.0, 0.0, 0.0});
   int err = 0;
   std::string err_msg;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -897,28 +897,14 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
           for (int r = 0; r < rows; ++r) P[off + a * rmax + r] = Pk[size_t(a) * rows + r];
         for (int r = 0; r < rows; ++r) R[roff + r] = c.st[k][r + 1].cell;
       }
-      // periodic translation codes of the members (read by the equilibrium
-      // averages of well-balanced runs)
+      // periodic translations of the members (read only by the equilibrium
+      // averages of well-balanced runs); coded after the loop
+      if (p.equilibrium == ZFV_EQ_NONE) continue;
       for (int k = 0; k < c.n; ++k) {
         int rows = c.size[k] - 1;
         int roff = (k == 0 ? 0 : s.rc + (k - 1) * s.rb);
         if (c.order[k] <= 1 || rows <= 0) continue;
-        for (int r = 0; r < rows; ++r) {
-          const V3 &t = c.st[k][r + 1].shift;
-          if (t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0) continue;
-          int code = -1;
-#pragma omp critical(zfv_shift_table)
-          {
-            for (int q = 1; q < int(shift_table.size()); ++q)
-              if (shift_table[q][0] == t[0] && shift_table[q][1] == t[1] && shift_table[q][2] == t[2]) code = q;
-            if (code < 0 && shift_table.size() < 16) {
-              shift_table.push_back(t);
-              code = int(shift_table.size()) - 1;
-            }
-          }
-          if (code < 0) throw ZfvError(ZFV_ERR_INVALID, "more than 15 distinct periodic translations in the stencils");
-          R[roff + r] |= int32_t(code) << 27;
-        }
+        for (int r = 0; r < rows; ++r) row_shift[size_t(i) * s.nrows + roff + r] = c.st[k][r + 1].shift;
       }
     } catch (const std::exception &e) {
 #pragma omp critical
@@ -931,8 +917,30 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   if (err) throw ZfvError(ZFV_ERR_INVALID, err_msg);
   img.max_cond = *std::max_element(cond.begin(), cond.end());
   if (n > (1 << 27)) throw ZfvError(ZFV_ERR_INVALID, "more than 2^27 cells");
-  for (size_t q = 0; q < shift_table.size(); ++q)
-    for (int d = 0; d < 3; ++d) img.shift[3 * q + d] = shift_table[q][d];
+  // Translation table and row codes, in caller order so that the same input
+  // always gives the same table.  Translations computed from different
+  // corner pairs agree to round-off: they are matched with a tolerance.
+  {
+    std::vector<V3> table(1, V3{0.0, 0.0, 0.0});
+    for (size_t q = 0; q < row_shift.size(); ++q) {
+      const V3 &t = row_shift[q];
+      if (t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0) continue;
+      const double tol = 1e-9 * std::max(1.0, std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+      int code = -1;
+      for (int k = 1; k < int(table.size()) && code < 0; ++k)
+        if (std::fabs(table[k][0] - t[0]) <= tol && std::fabs(table[k][1] - t[1]) <= tol &&
+            std::fabs(table[k][2] - t[2]) <= tol)
+          code = k;
+      if (code < 0) {
+        if (table.size() == 16) throw ZfvError(ZFV_ERR_INVALID, "more than 15 distinct periodic translations");
+        table.push_back(t);
+        code = int(table.size()) - 1;
+      }
+      rows_c[q] |= int32_t(code) << 27;
+    }
+    for (size_t q = 0; q < table.size(); ++q)
+      for (int d = 0; d < 3; ++d) img.shift[3 * q + d] = table[q][d];
+  }
 
   // reference-layout stencil export (caller numbering)
   img.l2g_off.assign(n + 1, 0);

@@ -14,6 +14,12 @@
 //   RK stage         src/zisa/ode/runge_kutta.cpp:87-143
 #pragma once
 
+// Rounding: everything here is compiled with -ffp-contract=off, so no
+// compiler decides where to fuse a multiply-add; the least-squares
+// accumulation and the polynomial evaluation use explicit fma().  The same
+// source therefore rounds identically in every kernel and in the host
+// emulation, whatever the surrounding code.
+
 #include <cstdint>
 
 #if defined(__HIPCC__)
@@ -242,7 +248,7 @@ ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, d
         const int i = (n + 1) * n / 2 + ky;
         const double b = pxk * pyk - c[i];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) px[v] += a[i][v] * b;
+        for (int v = 0; v < NV; ++v) px[v] = fma(a[i][v], b, px[v]);
       } else {
         double pzk = 1.0;
 #pragma unroll
@@ -253,7 +259,7 @@ ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, d
           const int i = d3 + (n2 + 1) * n2 / 2 + kz;
           const double b = pxk * pyk * pzk - c[i];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) px[v] += a[i][v] * b;
+          for (int v = 0; v < NV; ++v) px[v] = fma(a[i][v], b, px[v]);
           pzk *= X[2];
         }
       }
@@ -346,458 +352,26 @@ ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f)
   for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fL[v] + fR[v]) - 0.5 * s * (uR[v] - uL[v]);
 }
 
-// ---- the per-cell reconstruction --------------------------------------------------
+// ---- the per-cell reconstruction, one lane per conserved variable --------------
 //
 // Reads the cell's stencil, forms every stencil polynomial with the
 // precomputed pseudo-inverses, applies the CWENO correction and the WENO
 // weights, then evaluates the result at the quadrature points of the cell's
-// interior faces (rotated into each face's frame) and, for gravity, computes
+// interior faces (rotated into each face's frame) and, with gravity, computes
 // the cell's source term.
 //
-// U: state, nvars x n SoA.  fpv: [side][q][v] x ne.  src: nvars x n or null.
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
-ZFV_HD void reconstruct_cell(const Img &m, int64_t i, const double *__restrict__ U, double *__restrict__ fpv,
-                             double *__restrict__ src) {
-  constexpr int NV = DIM + 2;
-  constexpr int NF = DIM + 1;
-  constexpr int NC = DOFC - 1;   // central coefficients beyond the constant
-  constexpr int NBC = DIM;       // one-sided (linear) coefficients
-  constexpr int DEGC = deg_of<DIM>(DOFC);
-  const int64_t n = m.n;
-  const double gamma = m.gamma;
-
-  // own state and scale (include/zisa/model/characteristic_scale.hpp:23-32)
-  double u0[NV];
-  for (int v = 0; v < NV; ++v) u0[v] = U[v * n + i];
-  double msq = u0[1] * u0[1] + u0[2] * u0[2];
-  if (DIM == 3) msq += u0[3] * u0[3];
-  const double rho0 = u0[0];
-  const double Eint0 = u0[NV - 1] - 0.5 * msq / rho0;
-  const double cs = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
-  double scale[NV];
-  scale[0] = rho0;
-  for (int d = 0; d < DIM; ++d) scale[1 + d] = cs;
-  scale[NV - 1] = Eint0;
-  double inv[NV];
-  for (int v = 0; v < NV; ++v) inv[v] = 1.0 / scale[v];
-
-  double vtx[DIM + 1][3];
-  load_corners<DIM>(m, i, vtx);
-
-  // local equilibrium of this cell
-  double th[2] = {0.0, 0.0};
-  double phi_ref = 0.0;
-  const bool wb = m.eq != 0;
-  bool eq_found = false;
-  if (wb) {
-    double xref[3];
-    volume_point<DIM>(m, 0, vtx, xref);
-    phi_ref = phi<DIM>(m, xref);
-    double vol_i = m.vol[i];
-    // cell average of the equilibrium with parameters t over this cell
-    auto cell_avg = [&](const double *t, double &ar, double &aE) {
-      double sr = 0.0, sE = 0.0;
-      for (int q = 0; q < m.qv; ++q) {
-        double x[3];
-        volume_point<DIM>(m, q, vtx, x);
-        double r, E;
-        eq_eval<DIM>(m, t, phi_ref, x, r, E);
-        double w = vol_i * m.vw[q];
-        if (q == 0) {
-          sr = w * r;
-          sE = w * E;
-        } else {
-          sr = sr + w * r;
-          sE = sE + w * E;
-        }
-      }
-      ar = (1.0 * sr) / vol_i;
-      aE = (1.0 * sE) / vol_i;
-    };
-    if (m.eq == 2) {
-      double RT = (gamma - 1.0) * Eint0 / rho0;
-      double t1[2] = {1.0, RT};
-      double ar, aE;
-      cell_avg(t1, ar, aE);
-      th[0] = rho0 / ar;
-      th[1] = RT;
-      eq_found = true;
-    } else {
-      // quasi-Newton with a central finite-difference Jacobian
-      // (include/zisa/math/quasi_newton.hpp:12-50)
-      double p = Eint0 * (gamma - 1.0);
-      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / pow(rho0, gamma)};
-      double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
-      double x[2] = {guess[0], guess[1]};
-      auto f = [&](const double *t, double *fo) {
-        double ar, aE;
-        cell_avg(t, ar, aE);
-        fo[0] = rho0 - ar;
-        fo[1] = Eint0 - aE;
-      };
-      double fx[2];
-      f(x, fx);
-      double dx[2] = {2.0 * atol[0] + 1.0, 2.0 * atol[1] + 1.0};
-      double h0[2] = {0, 0}, h1[2] = {0, 0};  // previous two steps
-      bool ok = true;
-      int iter = 0;
-      while (!(fabs(dx[0]) <= atol[0] && fabs(dx[1]) <= atol[1]) && iter < 20) {
-        double J[2][2];
-        for (int dir = 0; dir < 2; ++dir) {
-          double e = 1e-6 * fabs(x[dir]);
-          double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]};
-          xp[dir] = x[dir] + 0.5 * e;
-          xm[dir] = x[dir] - 0.5 * e;
-          double fp[2], fm[2];
-          f(xp, fp);
-          f(xm, fm);
-          J[dir][0] = (fp[0] - fm[0]) / e;
-          J[dir][1] = (fp[1] - fm[1]) / e;
-        }
-        double inv_det = 1.0 / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-        dx[0] = inv_det * (J[1][1] * fx[0] - J[1][0] * fx[1]);
-        dx[1] = inv_det * (-J[0][1] * fx[0] + J[0][0] * fx[1]);
-        x[0] -= dx[0];
-        x[1] -= dx[1];
-        f(x, fx);
-        if (iter >= 4 && !(dx[0] <= atol[0] && dx[1] <= atol[1])) {
-          for (int k = 0; k < 2; ++k) {
-            double rate = log(fabs(dx[k]) / fabs(h1[k])) / log(fabs(h1[k]) / fabs(h0[k]));
-            if (!(rate >= 0.0)) ok = false;
-          }
-        }
-        h0[0] = h1[0];
-        h0[1] = h1[1];
-        h1[0] = dx[0];
-        h1[1] = dx[1];
-        if (!ok) break;
-        ++iter;
-      }
-      if (ok && !(iter == 20 && !(fabs(dx[0]) <= 1000.0 * atol[0] && fabs(dx[1]) <= 1000.0 * atol[1]))) {
-        th[0] = x[0];
-        th[1] = x[1];
-        eq_found = true;
-      }
-    }
-  }
-
-  // equilibrium average of this cell's equilibrium over cell j (shifted)
-  auto member_eq = [&](int64_t j, int code, double &er, double &eE) {
-    er = 0.0;
-    eE = 0.0;
-    if (!wb || !eq_found) return;
-    double w3[DIM + 1][3];
-    load_corners<DIM>(m, j, w3);
-    for (int k = 0; k < DIM + 1; ++k)
-      for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
-    double vol_j = m.vol[j];
-    double sr = 0.0, sE = 0.0;
-    for (int q = 0; q < m.qv; ++q) {
-      double x[3];
-      volume_point<DIM>(m, q, w3, x);
-      double r, E;
-      eq_eval<DIM>(m, th, phi_ref, x, r, E);
-      double w = vol_j * m.vw[q];
-      if (q == 0) {
-        sr = w * r;
-        sE = w * E;
-      } else {
-        sr = sr + w * r;
-        sE = sE + w * E;
-      }
-    }
-    er = (1.0 * sr) / vol_j;
-    eE = (1.0 * sE) / vol_j;
-  };
-
-  double q0[NV];
-  {
-    double er, eE;
-    member_eq(i, 0, er, eE);
-    for (int v = 0; v < NV; ++v) q0[v] = u0[v];
-    q0[0] -= er;
-    q0[NV - 1] -= eE;
-    for (int v = 0; v < NV; ++v) q0[v] = q0[v] * inv[v];
-  }
-
-  const int32_t meta = m.meta[i];
-  const bool first_order = (meta & 8) != 0;
-  const int kh = meta & 7;
-
-  // polynomial coefficients: central c[alpha][v] (alpha = 1..DOFC-1),
-  // one-sided b[k][alpha][v] (alpha = 1..DIM); constants a0 per stencil.
-  double c[NC][NV];
-  double b[NB > 0 ? NB : 1][NBC][NV];
-  for (int a = 0; a < NC; ++a)
-    for (int v = 0; v < NV; ++v) c[a][v] = 0.0;
-  for (int k = 0; k < NB; ++k)
-    for (int a = 0; a < NBC; ++a)
-      for (int v = 0; v < NV; ++v) b[k][a][v] = 0.0;
-
-  if (!first_order) {
-    // central stencil: rows 0 .. RC-1
-#pragma unroll 2
-    for (int r = 0; r < RC; ++r) {
-      const int32_t rw = m.rows[r * n + i];
-      const int64_t j = rw & ROW_CELL_MASK;
-      double d[NV];
-      for (int v = 0; v < NV; ++v) d[v] = U[v * n + j];
-      if (wb) {
-        double er, eE;
-        member_eq(j, rw >> ROW_CELL_BITS, er, eE);
-        d[0] -= er;
-        d[NV - 1] -= eE;
-      }
-      for (int v = 0; v < NV; ++v) d[v] = d[v] * inv[v] - q0[v];
-#pragma unroll
-      for (int a = 0; a < NC; ++a) {
-        const double p = m.P[(a * RC + r) * n + i];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) c[a][v] += p * d[v];
-      }
-    }
-    // one-sided stencils
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      const int off = NC * RC + k * NBC * RB;
-      const int roff = RC + k * RB;
-#pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        const int32_t rw = m.rows[(roff + r) * n + i];
-        const int64_t j = rw & ROW_CELL_MASK;
-        double d[NV];
-        for (int v = 0; v < NV; ++v) d[v] = U[v * n + j];
-        if (wb) {
-          double er, eE;
-          member_eq(j, rw >> ROW_CELL_BITS, er, eE);
-          d[0] -= er;
-          d[NV - 1] -= eE;
-        }
-        for (int v = 0; v < NV; ++v) d[v] = d[v] * inv[v] - q0[v];
-#pragma unroll
-        for (int a = 0; a < NBC; ++a) {
-          const double p = m.P[(off + a * RB + r) * n + i];
-#pragma unroll
-          for (int v = 0; v < NV; ++v) b[k][a][v] += p * d[v];
-        }
-      }
-    }
-  }
-
-  // final polynomial a[alpha][v], alpha < DOFC
-  double a[DOFC][NV];
-  if (first_order) {
-    // one constant stencil with weight 1 (make_reconstruction_array o1 params)
-    for (int v = 0; v < NV; ++v) {
-      // p = 0 + (al/al) * p0, p0 = (p0)/1 for CWENO
-      double w = 1.0;
-      a[0][v] = 0.0 + w * q0[v];
-    }
-    for (int al = 1; al < DOFC; ++al)
-      for (int v = 0; v < NV; ++v) a[al][v] = 0.0;
-  } else {
-    double a0[NB + 1][NV];
-    for (int k = 0; k <= NB; ++k)
-      for (int v = 0; v < NV; ++v) a0[k][v] = q0[v];
-    if (m.mode == 0) {
-      // CWENO: p_hi <- (p_hi - sum_{k != hi} c_k p_k) * (1 / c_hi)
-      if (kh == 0) {
-        for (int k = 1; k <= NB; ++k) {
-          const double w = m.lw[k];
-          for (int v = 0; v < NV; ++v) {
-            a0[0][v] = a0[0][v] - w * a0[k][v];
-            for (int al = 0; al < NBC; ++al) c[al][v] = c[al][v] - w * b[k - 1][al][v];
-          }
-        }
-        const double iw = 1.0 / m.lw[0];
-        for (int v = 0; v < NV; ++v) {
-          a0[0][v] = iw * a0[0][v];
-          for (int al = 0; al < NC; ++al) c[al][v] = iw * c[al][v];
-        }
-      } else if (NB > 0) {
-        // the corrected polynomial is a one-sided one (central stencil reduced to order 1)
-        const int hb = kh - 1;
-        for (int k = 0; k <= NB; ++k) {
-          if (k == kh) continue;
-          const double w = m.lw[k];
-          for (int v = 0; v < NV; ++v) {
-            a0[kh][v] = a0[kh][v] - w * a0[k][v];
-            for (int al = 0; al < NBC; ++al) {
-              double o = (k == 0) ? c[al][v] : b[k - 1][al][v];
-              b[hb][al][v] = b[hb][al][v] - w * o;
-            }
-          }
-        }
-        const double iw = 1.0 / m.lw[kh];
-        for (int v = 0; v < NV; ++v) {
-          a0[kh][v] = iw * a0[kh][v];
-          for (int al = 0; al < NBC; ++al) b[hb][al][v] = iw * b[hb][al][v];
-        }
-      }
-    }
-    // smoothness indicators and nonlinear weights (hybrid_weno.cpp:110-127)
-    double alpha[NB + 1];
-    double al_tot = 0.0;
-    {
-      double IS = 0.0;
-      for (int v = 0; v < NV; ++v) {
-        double s = 0.0;
-        for (int al = 0; al < NC; ++al) s += c[al][v] * c[al][v];
-        IS = v == 0 ? s : dmax(IS, s);
-      }
-      double isp = m.exponent == 4.0 ? sqr(sqr(IS)) : pow(IS, m.exponent);
-      alpha[0] = m.lw[0] / (m.eps + isp);
-      al_tot += alpha[0];
-    }
-    for (int k = 0; k < NB; ++k) {
-      double IS = 0.0;
-      for (int v = 0; v < NV; ++v) {
-        double s = 0.0;
-        for (int al = 0; al < NBC; ++al) s += b[k][al][v] * b[k][al][v];
-        IS = v == 0 ? s : dmax(IS, s);
-      }
-      double isp = m.exponent == 4.0 ? sqr(sqr(IS)) : pow(IS, m.exponent);
-      alpha[k + 1] = m.lw[k + 1] / (m.eps + isp);
-      al_tot += alpha[k + 1];
-    }
-    for (int v = 0; v < NV; ++v) {
-      const double w0 = alpha[0] / al_tot;
-      a[0][v] = 0.0 + w0 * a0[0][v];
-      for (int al = 1; al < DOFC; ++al) a[al][v] = 0.0 + w0 * c[al - 1][v];
-      for (int k = 0; k < NB; ++k) {
-        const double w = alpha[k + 1] / al_tot;
-        a[0][v] = a[0][v] + w * a0[k + 1][v];
-        for (int al = 1; al <= NBC; ++al) a[al][v] = a[al][v] + w * b[k][al - 1][v];
-      }
-    }
-  }
-
-  // moments: zero for alpha < 3
-  double cm[DOFC];
-  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
-  double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
-  const double len = m.len[i];
-  const uint32_t fperm = m.fperm[i];
-
-  // scaled reconstruction (+ equilibrium background) at x
-  auto eval = [&](const double *x, double *u, double *delta) {
-    double X[3] = {(x[0] - xc[0]) / len, (x[1] - xc[1]) / len, DIM == 3 ? (x[2] - xc[2]) / len : 0.0};
-    double px[NV];
-    poly_eval<DIM, DEGC, NV>(a, cm, X, px);
-    for (int v = 0; v < NV; ++v) delta[v] = scale[v] * px[v];
-    for (int v = 0; v < NV; ++v) u[v] = delta[v];
-    if (wb && eq_found) {
-      double r, E;
-      eq_eval<DIM>(m, th, phi_ref, x, r, E);
-      u[0] = r + delta[0];
-      for (int d = 0; d < DIM; ++d) u[1 + d] = 0.0 + delta[1 + d];
-      u[NV - 1] = E + delta[NV - 1];
-    }
-  };
-
-  double s_acc[NV];
-  for (int v = 0; v < NV; ++v) s_acc[v] = 0.0;
-  const bool gravity_src = (m.gravity != 0) && (src != nullptr);
-
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    const int32_t cf = m.cface[f * n + i];
-    double w3[3][3];
-    face_vertices<DIM>(fperm, f, vtx, w3);
-    double nn[3], t1[3], t2[3], area;
-    face_frame<DIM>(w3, nn, t1, t2, area);
-    const bool exterior = (cf & 2) != 0;
-    // outward normal sign for the source term (src/zisa/math/face.cpp:24-27)
-    double x0f[3];
-    face_point<DIM>(m, 0, w3, x0f);
-    double sgn = 0.0;
-    if (gravity_src && wb) {
-      double dd = nn[0] * (x0f[0] - xc[0]) + nn[1] * (x0f[1] - xc[1]) + (DIM == 3 ? nn[2] * (x0f[2] - xc[2]) : 0.0);
-      sgn = double((0.0 < dd) - (dd < 0.0));
-    }
-    double fs[NV];
-    for (int v = 0; v < NV; ++v) fs[v] = 0.0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      double x[3];
-      face_point<DIM>(m, q, w3, x);
-      if (!exterior) {
-        double u[NV], dl[NV];
-        eval(x, u, dl);
-        // rotate into the face frame
-        double r[NV];
-        r[0] = u[0];
-        r[NV - 1] = u[NV - 1];
-        if (DIM == 2) {
-          r[1] = u[1] * nn[0] + u[2] * nn[1];
-          r[2] = u[1] * t1[0] + u[2] * t1[1];
-        } else {
-          r[1] = u[1] * nn[0] + u[2] * nn[1] + u[3] * nn[2];
-          r[2] = u[1] * t1[0] + u[2] * t1[1] + u[3] * t1[2];
-          r[3] = u[1] * t2[0] + u[2] * t2[1] + u[3] * t2[2];
-        }
-        const int64_t e = cf >> 2;
-        const int side = cf & 1;
-        for (int v = 0; v < NV; ++v) fpv[((side * Q + q) * NV + v) * m.ne + e] = r[v];
-      }
-      if (gravity_src && wb && eq_found) {
-        double r, E;
-        eq_eval<DIM>(m, th, phi_ref, x, r, E);
-        double p_eq = E * (gamma - 1.0);
-        double w = area * m.fw[q];
-        for (int d = 0; d < DIM; ++d) {
-          double t = p_eq * (sgn * nn[d]);
-          fs[1 + d] = (q == 0) ? w * t : fs[1 + d] + w * t;
-        }
-      }
-    }
-    if (gravity_src && wb)
-      for (int v = 0; v < NV; ++v) s_acc[v] += 1.0 * fs[v];
-  }
-
-  if (gravity_src) {
-    const double vol_i = m.vol[i];
-    double cs_[NV];
-    for (int v = 0; v < NV; ++v) cs_[v] = 0.0;
-    for (int q = 0; q < m.qv; ++q) {
-      double x[3];
-      volume_point<DIM>(m, q, vtx, x);
-      double u[NV], dl[NV];
-      eval(x, u, dl);
-      double gp[3];
-      grad_phi<DIM>(m, x, gp);
-      double drho = wb ? dl[0] : u[0];
-      double sv[NV];
-      sv[0] = 0.0;
-      double mg = 0.0;
-      for (int d = 0; d < DIM; ++d) {
-        sv[1 + d] = -drho * gp[d];
-        mg += u[1 + d] * gp[d];
-      }
-      sv[NV - 1] = -mg;
-      double w = vol_i * m.vw[q];
-      for (int v = 0; v < NV; ++v) cs_[v] = (q == 0) ? w * sv[v] : cs_[v] + w * sv[v];
-    }
-    for (int v = 0; v < NV; ++v) {
-      double sum = wb ? (s_acc[v] + 1.0 * cs_[v]) : 1.0 * cs_[v];
-      src[v * n + i] = sum / vol_i;
-    }
-  }
-}
-
-// ---- the per-cell reconstruction, one lane per conserved variable --------------
-//
-// The same computation as reconstruct_cell, spread over a group of NV lanes
-// that share one cell: lane g owns variable g (rho, momentum components,
-// energy).  Each lane keeps only its variable's polynomial coefficients, so a
-// lane needs a fraction of the registers of a whole cell.  The lanes exchange
-// values through X::bcast(v, k) (value v of lane k of the same group):
+// A group of NV lanes shares one cell: lane g owns conserved variable g (rho,
+// momentum components, energy) and keeps only that variable's polynomial
+// coefficients, so a lane needs a fraction of the registers of a whole cell.
+// The lanes exchange values through X::bcast(v, k) (value v of lane k of the
+// same group):
 //   - the equilibrium averages of the stencil members are computed by lane
 //     (row mod NV) and broadcast to the density and energy lanes,
 //   - the smoothness indicator of a stencil is the maximum over the variables,
 //   - the face-frame rotation and the gravity source mix momentum components.
 // Every call of bcast sits in control flow that is uniform over the group.
-// Arithmetic per variable follows reconstruct_cell operation for operation.
+//
+// U: state, nvars x n SoA.  fpv: [side][q][v] x ne.  src: nvars x n or null.
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, class X>
 ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const double *__restrict__ U,
                              double *__restrict__ fpv, double *__restrict__ src) {
@@ -1007,7 +581,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           d = d * inv_g - q0;
           if (r < RC) {
 #pragma unroll
-            for (int a = 0; a < NC; ++a) c[a] += m.P[(a * RC + r) * n + i] * d;
+            for (int a = 0; a < NC; ++a) c[a] = fma(m.P[(a * RC + r) * n + i], d, c[a]);
           } else {
             const int kb = (r - RC) / RB;
             const int rr = (r - RC) - kb * RB;
@@ -1017,7 +591,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
               const double p = m.P[(off + a * RB + rr) * n + i];
 #pragma unroll
               for (int kk = 0; kk < NB; ++kk)
-                if (kk == kb) b[kk][a] += p * d;
+                if (kk == kb) b[kk][a] = fma(p, d, b[kk][a]);
             }
           }
         }

@@ -174,6 +174,35 @@ def batched_2d(n_instances=64, min_cells=5000, max_cells=200000, seed=4, order=3
     return out
 
 
+def serving_requests(n_requests=16, seed=5, min_tris=10000, max_tris=200000, min_tets=10000, max_tets=60000,
+                     t_min=0.05, t_max=0.5):
+    """Solve requests for the serving runtime: every fourth a 3D polytrope
+    (tetrahedra log-uniform in [min_tets, max_tets]), the others 2D, half
+    isentropic vortices and half isothermal atmospheres with a pressure bump
+    (triangles log-uniform in [min_tris, max_tris]); end times U[t_min, t_max];
+    CWENO-AO order 3 throughout."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_requests):
+        t_end = float(rng.uniform(t_min, t_max))
+        if k % 4 == 3:
+            tets = np.exp(rng.uniform(np.log(min_tets), np.log(max_tets)))
+            # Kuhn cubes inside the unit ball: about 6 (n/2)^3 (4/3) pi tetrahedra
+            n = max(8, int(round(2.0 * (tets / (8.0 * np.pi)) ** (1.0 / 3.0))))
+            wl = polytrope_3d(n=n, amplitude=float(10 ** rng.uniform(-7, -3)), seed=2000 + k)
+            wl.n_steps = 0
+        else:
+            tris = np.exp(rng.uniform(np.log(min_tris), np.log(max_tris)))
+            n = max(8, int(round(np.sqrt(tris / 2.0))))
+            if k % 2 == 0:
+                wl = vortex_2d(n=n, beta=float(rng.uniform(2.0, 6.0)), seed=3000 + k)
+            else:
+                wl = isothermal_2d(n=n, order=3, amplitude=float(10 ** rng.uniform(-7, -3)), seed=3000 + k)
+        wl.t_end = t_end
+        out.append(wl)
+    return out
+
+
 def polytrope_3d(n=40, order=3, amplitude=1e-5, seed=3, jitter=0.1, radius=1.0, ghost_radius=0.85) -> Workload:
     """Well-balanced 3D: n = 1 polytrope in the unit ball, radial gravity from
     the enclosed mass, Rusanov flux, isentropic equilibrium."""
