@@ -13,11 +13,15 @@ captured HIP graph with the state resident in HBM.  value = n_cells * 3 * K / T
 over exactly K timed steps.  The JSON line also carries
   - roofline: the dominant kernel (k_reconstruct), algorithmic bytes per launch
     / its average HIP-event duration on the handle's stream;
-  - batched_vs_plain: the batched configuration (64 members, 5k-200k cells,
-    order 3, per-member CFL every step, one graph per step for all members)
-    against the same members stepped one after another on the plain path;
   - cpu_baseline: the oracle (the reference's OpenMP CPU path restated) timed
     on this host's cores over a bounded sample.
+Opt-in legs:
+  --batch  batched_vs_plain: the batched configuration (64 members, 5k-200k
+           cells, order 3, per-member CFL every step, one graph per step for
+           all members) against the same members stepped one after another on
+           the plain path;
+  --serve  serving: Poisson request streams through the serving runtime,
+           p50/p99 latency and aggregate cell-stage updates/s.
 """
 from __future__ import annotations
 
@@ -195,9 +199,10 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="add the batched-vs-plain leg (64 members)")
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="add the serving leg (Poisson request streams)")
     ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=1.0)
@@ -273,11 +278,11 @@ def main():
         "measured_copy_gbs": z.bandwidth_probe(1 << 30, 10),
     }
     s.close()
-    if not args.no_batch:
+    if args.batch:
         log("batched vs plain ...")
         out["batched_vs_plain"] = batched_vs_plain(z, args.batch_steps)
         log(json.dumps(out["batched_vs_plain"]))
-    if not args.no_serve:
+    if args.serve:
         log("serving ...")
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         out["serving"] = serving(z, rates, args.serve_requests, args.serve_budget_gib)
