@@ -5,6 +5,7 @@
 // (zisafvm_amd/csrc/stage_math.hpp) in host loops, so the CPU test suite can
 // check the product's numerics against the oracle without a GPU.  The
 // product library never loads this; GPU tests call libzfv_hip.so.
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -37,27 +38,30 @@ struct Emu {
 // barrier then times out and the emulation reports the error.
 struct LaneBarrier {
   int n;
-  std::mutex mu;
-  std::condition_variable cv;
-  int count = 0;
-  long gen = 0;
-  bool broken = false;
+  std::atomic<int> count{0};
+  std::atomic<long> gen{0};
+  std::atomic<bool> broken{false};
   double slot[8] = {0};
   explicit LaneBarrier(int n_) : n(n_) {}
   void wait() {
-    std::unique_lock<std::mutex> lk(mu);
-    if (broken) throw std::runtime_error("lane group broken");
-    const long my = gen;
-    if (++count == n) {
-      count = 0;
-      ++gen;
-      cv.notify_all();
+    if (broken.load()) throw std::runtime_error("lane group broken");
+    const long my = gen.load(std::memory_order_acquire);
+    if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+      count.store(0, std::memory_order_relaxed);
+      gen.fetch_add(1, std::memory_order_acq_rel);
       return;
     }
-    if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return gen != my || broken; }) || broken) {
-      broken = true;
-      cv.notify_all();
-      throw std::runtime_error("lane group: bcast calls differ between lanes");
+    const auto t0 = std::chrono::steady_clock::now();
+    long spins = 0;
+    while (gen.load(std::memory_order_acquire) == my) {
+      if (broken.load()) throw std::runtime_error("lane group broken");
+      if (++spins % 1024 == 0) {
+        std::this_thread::yield();
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+          broken = true;
+          throw std::runtime_error("lane group: bcast calls differ between lanes");
+        }
+      }
     }
   }
 };

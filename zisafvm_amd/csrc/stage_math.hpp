@@ -409,20 +409,31 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     volume_point<DIM>(m, 0, vtx, xref);
     phi_ref = phi<DIM, PH>(m, xref);
     double vol_i = m.vol[i];
+    // cell average of the equilibrium with parameters t over this cell: lane
+    // k evaluates the points q = k (mod NV), the sum runs in point order in
+    // every lane
     auto cell_avg = [&](const double *t, double &ar, double &aE) {
       double sr = 0.0, sE = 0.0;
-      for (int q = 0; q < m.qv; ++q) {
-        double x[3];
-        volume_point<DIM>(m, q, vtx, x);
-        double r, E;
-        eq_eval<DIM, PH>(m, t, phi_ref, x, r, E);
-        double w = vol_i * m.vw[q];
-        if (q == 0) {
-          sr = w * r;
-          sE = w * E;
-        } else {
-          sr = sr + w * r;
-          sE = sE + w * E;
+      for (int q0 = 0; q0 < m.qv; q0 += NV) {
+        double my_r = 0.0, my_E = 0.0;
+        if (q0 + g < m.qv) {
+          double x[3];
+          volume_point<DIM>(m, q0 + g, vtx, x);
+          eq_eval<DIM, PH>(m, t, phi_ref, x, my_r, my_E);
+        }
+        for (int k = 0; k < NV; ++k) {
+          const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
+          const int q = q0 + k;
+          if (q < m.qv) {
+            double w = vol_i * m.vw[q];
+            if (q == 0) {
+              sr = w * r;
+              sE = w * E;
+            } else {
+              sr = sr + w * r;
+              sE = sE + w * E;
+            }
+          }
         }
       }
       ar = (1.0 * sr) / vol_i;
@@ -526,11 +537,41 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     eE = (1.0 * sE) / vol_j;
   };
   const bool eq_sub = wb && eq_found;
+  // this cell's equilibrium averaged over itself (member_eq(i, 0) with the
+  // points split over the group's lanes)
+  auto own_eq = [&](double &er, double &eE) {
+    const double vol_i = m.vol[i];
+    double sr = 0.0, sE = 0.0;
+    for (int q0 = 0; q0 < m.qv; q0 += NV) {
+      double my_r = 0.0, my_E = 0.0;
+      if (q0 + g < m.qv) {
+        double x[3];
+        volume_point<DIM>(m, q0 + g, vtx, x);
+        eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
+      }
+      for (int k = 0; k < NV; ++k) {
+        const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
+        const int q = q0 + k;
+        if (q < m.qv) {
+          double w = vol_i * m.vw[q];
+          if (q == 0) {
+            sr = w * r;
+            sE = w * E;
+          } else {
+            sr = sr + w * r;
+            sE = sE + w * E;
+          }
+        }
+      }
+    }
+    er = (1.0 * sr) / vol_i;
+    eE = (1.0 * sE) / vol_i;
+  };
 
   double q0;
   {
     double er = 0.0, eE = 0.0;
-    if (eq_sub) member_eq(i, 0, er, eE);
+    if (eq_sub) own_eq(er, eE);
     q0 = u0[g];
     if (is_rho) q0 -= er;
     if (is_E) q0 -= eE;
@@ -678,20 +719,15 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   const double len = m.len[i];
   const uint32_t fperm = m.fperm[i];
 
-  // this variable of the scaled reconstruction (+ equilibrium background) at
-  // x; the equilibrium point values come back in r_eq, E_eq
-  auto eval = [&](const double *x, double &u, double &delta, double &r_eq, double &E_eq) {
+  // this variable of the scaled reconstruction at x, given the equilibrium
+  // point values (r_eq, E_eq) there; delta is the scaled deviation
+  auto eval = [&](const double *x, double r_eq, double E_eq, double &u, double &delta) {
     double Xs[3] = {(x[0] - xc[0]) / len, (x[1] - xc[1]) / len, DIM == 3 ? (x[2] - xc[2]) / len : 0.0};
     double px;
     poly_eval<DIM, DEGC, 1>(reinterpret_cast<const double(*)[1]>(a), cm, Xs, &px);
     delta = scale_g * px;
     u = delta;
-    r_eq = 0.0;
-    E_eq = 0.0;
-    if (eq_sub) {
-      eq_eval<DIM, PH>(m, th, phi_ref, x, r_eq, E_eq);
-      u = is_rho ? r_eq + delta : (is_E ? E_eq + delta : 0.0 + delta);
-    }
+    if (eq_sub) u = is_rho ? r_eq + delta : (is_E ? E_eq + delta : 0.0 + delta);
   };
 
   double s_acc = 0.0;
@@ -713,15 +749,38 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double dd = nn[0] * (x0f[0] - xc[0]) + nn[1] * (x0f[1] - xc[1]) + (DIM == 3 ? nn[2] * (x0f[2] - xc[2]) : 0.0);
       sgn = double((0.0 < dd) - (dd < 0.0));
     }
+    // equilibrium point values of this face: lane k evaluates point q = k
+    // (mod NV), every lane receives all of them
+    double eqr[Q], eqE[Q];
+    for (int q = 0; q < Q; ++q) eqr[q] = eqE[q] = 0.0;
+    const bool need_eq = eq_sub && (!exterior || gravity_src);
+    if (need_eq) {
+#pragma unroll
+      for (int q0 = 0; q0 < Q; q0 += NV) {
+        double my_r = 0.0, my_E = 0.0;
+        if (q0 + g < Q) {
+          double x[3];
+          face_point<DIM>(m, q0 + g, w3, x);
+          eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
+          if (q0 + k < Q) {
+            eqr[q0 + k] = r;
+            eqE[q0 + k] = E;
+          }
+        }
+      }
+    }
     double fs = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      double x[3];
-      face_point<DIM>(m, q, w3, x);
-      double r_eq = 0.0, E_eq = 0.0;
       if (!exterior) {
+        double x[3];
+        face_point<DIM>(m, q, w3, x);
         double u, dl;
-        eval(x, u, dl, r_eq, E_eq);
+        eval(x, eqr[q], eqE[q], u, dl);
         // rotate into the face frame: momentum lanes need every component
         double mc[3];
         for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
@@ -732,11 +791,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         const int64_t e = cf >> 2;
         const int side = cf & 1;
         fpv[((side * Q + q) * NV + g) * m.ne + e] = r;
-      } else if (gravity_src && eq_sub) {
-        eq_eval<DIM, PH>(m, th, phi_ref, x, r_eq, E_eq);
       }
       if (gravity_src && eq_sub && g >= 1 && g <= DIM) {
-        double p_eq = E_eq * (gamma - 1.0);
+        double p_eq = eqE[q] * (gamma - 1.0);
         double w = area * m.fw[q];
         double t = p_eq * (sgn * nn[g - 1]);
         fs = (q == 0) ? w * t : fs + w * t;
@@ -748,25 +805,41 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   if (gravity_src) {
     const double vol_i = m.vol[i];
     double cs_ = 0.0;
-    for (int q = 0; q < m.qv; ++q) {
-      double x[3];
-      volume_point<DIM>(m, q, vtx, x);
-      double u, dl, r_eq, E_eq;
-      eval(x, u, dl, r_eq, E_eq);
-      double gp[3];
-      grad_phi<DIM, PH>(m, x, gp);
-      const double drho = xch.bcast(wb ? dl : u, 0);
-      double mc[3];
-      for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
-      double sv = 0.0;
-      if (g >= 1 && g <= DIM) sv = -drho * gp[g - 1];
-      if (is_E) {
-        double mg = 0.0;
-        for (int d = 0; d < DIM; ++d) mg += mc[d] * gp[d];
-        sv = -mg;
+    for (int q0 = 0; q0 < m.qv; q0 += NV) {
+      // equilibrium values at the points q0 .. q0 + NV - 1, one per lane
+      double my_r = 0.0, my_E = 0.0;
+      if (eq_sub && q0 + g < m.qv) {
+        double x[3];
+        volume_point<DIM>(m, q0 + g, vtx, x);
+        eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
       }
-      double w = vol_i * m.vw[q];
-      cs_ = (q == 0) ? w * sv : cs_ + w * sv;
+      for (int k = 0; k < NV; ++k) {
+        const int q = q0 + k;
+        double r_eq = 0.0, E_eq = 0.0;
+        if (eq_sub) {
+          r_eq = xch.bcast(my_r, k);
+          E_eq = xch.bcast(my_E, k);
+        }
+        if (q >= m.qv) break;  // uniform over the group
+        double x[3];
+        volume_point<DIM>(m, q, vtx, x);
+        double u, dl;
+        eval(x, r_eq, E_eq, u, dl);
+        double gp[3];
+        grad_phi<DIM, PH>(m, x, gp);
+        const double drho = xch.bcast(wb ? dl : u, 0);
+        double mc[3];
+        for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
+        double sv = 0.0;
+        if (g >= 1 && g <= DIM) sv = -drho * gp[g - 1];
+        if (is_E) {
+          double mg = 0.0;
+          for (int d = 0; d < DIM; ++d) mg += mc[d] * gp[d];
+          sv = -mg;
+        }
+        double w = vol_i * m.vw[q];
+        cs_ = (q == 0) ? w * sv : cs_ + w * sv;
+      }
     }
     double sum = wb ? (s_acc + 1.0 * cs_) : 1.0 * cs_;
     src[g * n + i] = sum / vol_i;
