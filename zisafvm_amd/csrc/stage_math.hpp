@@ -149,6 +149,45 @@ ZFV_HD void eq_eval(const Img &m, const double *th, double phi_ref, const double
   }
 }
 
+// The same equilibrium with its per-cell constants hoisted out of the point
+// loops: isentropic (the reference's): a = (1/K)(gamma-1)/gamma and
+// b = 1/(gamma-1), the very subexpressions eq_eval forms at every point, so
+// the values are identical; isothermal (an extension): a = -1/RT and
+// b = RT/(gamma-1), which replace two divisions per point by products.
+struct EqC {
+  double th0, th1, phi_ref, a, b;
+};
+
+template <int PH>
+ZFV_HD EqC eq_consts(const Img &m, const double *th, double phi_ref) {
+  EqC c;
+  c.th0 = th[0];
+  c.th1 = th[1];
+  c.phi_ref = phi_ref;
+  if (PH == PH_FULL && m.eq == 1) {
+    c.a = 1.0 / th[1] * (m.gamma - 1.0) / m.gamma;
+    c.b = 1.0 / (m.gamma - 1.0);
+  } else {
+    c.a = -1.0 / th[1];
+    c.b = th[1] / (m.gamma - 1.0);
+  }
+  return c;
+}
+
+template <int DIM, int PH>
+ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, double &E) {
+  const double ph = phi<DIM, PH>(m, x);
+  if (PH == PH_FULL && m.eq == 1) {
+    double h = c.th0 + c.phi_ref - ph;
+    rho = pow(c.a * h, c.b);
+    double p = c.th1 * pow(rho, m.gamma);
+    E = p / (m.gamma - 1.0);
+  } else {
+    rho = c.th0 * exp((ph - c.phi_ref) * c.a);
+    E = rho * c.b;
+  }
+}
+
 // ---- geometry helpers ---------------------------------------------------------
 template <int DIM>
 ZFV_HD void load_corners(const Img &m, int64_t i, double (*v)[3]) {
@@ -225,11 +264,12 @@ ZFV_HD void face_point(const Img &m, int q, const double (*w)[3], double *x) {
 
 template <int DIM>
 ZFV_HD void volume_point(const Img &m, int q, const double (*v)[3], double *x) {
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < DIM; ++d) {
     double s = m.vx[q][0] * v[0][d];
     for (int k = 1; k < DIM + 1; ++k) s = s + m.vx[q][k] * v[k][d];
     x[d] = s;
   }
+  if (DIM == 2) x[2] = 0.0;
 }
 
 // ---- polynomial evaluation (include/zisa/math/poly2d_impl.hpp:271-323) -------
@@ -413,13 +453,14 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // k evaluates the points q = k (mod NV), the sum runs in point order in
     // every lane
     auto cell_avg = [&](const double *t, double &ar, double &aE) {
+      const EqC tc = eq_consts<PH>(m, t, phi_ref);
       double sr = 0.0, sE = 0.0;
       for (int q0 = 0; q0 < m.qv; q0 += NV) {
         double my_r = 0.0, my_E = 0.0;
         if (q0 + g < m.qv) {
           double x[3];
           volume_point<DIM>(m, q0 + g, vtx, x);
-          eq_eval<DIM, PH>(m, t, phi_ref, x, my_r, my_E);
+          eq_point<DIM, PH>(m, tc, x, my_r, my_E);
         }
         for (int k = 0; k < NV; ++k) {
           const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
@@ -511,6 +552,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     }
   }
 
+  const EqC thc = eq_consts<PH>(m, th, phi_ref);
+
   // equilibrium average of this cell's equilibrium over cell j (shifted)
   auto member_eq = [&](int64_t j, int code, double &er, double &eE) {
     double w3[DIM + 1][3];
@@ -523,7 +566,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double x[3];
       volume_point<DIM>(m, q, w3, x);
       double r, E;
-      eq_eval<DIM, PH>(m, th, phi_ref, x, r, E);
+      eq_point<DIM, PH>(m, thc, x, r, E);
       double w = vol_j * m.vw[q];
       if (q == 0) {
         sr = w * r;
@@ -547,7 +590,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (q0 + g < m.qv) {
         double x[3];
         volume_point<DIM>(m, q0 + g, vtx, x);
-        eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
+        eq_point<DIM, PH>(m, thc, x, my_r, my_E);
       }
       for (int k = 0; k < NV; ++k) {
         const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
@@ -761,7 +804,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         if (q0 + g < Q) {
           double x[3];
           face_point<DIM>(m, q0 + g, w3, x);
-          eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
+          eq_point<DIM, PH>(m, thc, x, my_r, my_E);
         }
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
@@ -811,7 +854,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (eq_sub && q0 + g < m.qv) {
         double x[3];
         volume_point<DIM>(m, q0 + g, vtx, x);
-        eq_eval<DIM, PH>(m, th, phi_ref, x, my_r, my_E);
+        eq_point<DIM, PH>(m, thc, x, my_r, my_E);
       }
       for (int k = 0; k < NV; ++k) {
         const int q = q0 + k;
