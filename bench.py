@@ -62,6 +62,18 @@ def build_workload(name, n):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload, n_cells):
+    """HBM bytes per k_reconstruct launch from the committed PMC summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
+    reads for a copy of known size), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)[f"k_reconstruct|{workload}|{n_cells}"]
+        return t["fetch_bytes"] + t["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(wl, budget_s):
     """Oracle rate-of-change calls on this host's cores for about budget_s."""
     from oracle.zfv_ref import Oracle
@@ -269,9 +281,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(wl.name, n),
             "kernel": "k_reconstruct",
             "bytes_per_cell": info.recon_bytes_per_cell,
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE of this workload, "
+                              "scaled by the copy-kernel calibration)",
         },
         "setup_s": t_setup,
         "tuning": s.tune(-1).as_dict(),
