@@ -42,6 +42,7 @@ struct LaneBarrier {
   std::atomic<long> gen{0};
   std::atomic<bool> broken{false};
   double slot[8] = {0};
+  double lds[64] = {0};  // the group's LDS slice
   explicit LaneBarrier(int n_) : n(n_) {}
   void wait() {
     if (broken.load()) throw std::runtime_error("lane group broken");
@@ -76,6 +77,9 @@ struct EmuXch {
     b->wait();
     return r;
   }
+  int32_t bcast_i(int32_t v, int k) { return int32_t(bcast(double(v), k)); }
+  double *lds() { return b->lds; }
+  void sync() { b->wait(); }
 };
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>

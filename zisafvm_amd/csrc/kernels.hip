@@ -37,11 +37,32 @@ static thread_local std::string g_last_error;
 // ---------------------------------------------------------------------------
 // kernels
 
+// Blocks are dispatched to the 8 XCDs round robin (block b to XCD b mod 8).
+// The logical block of hardware block b makes every XCD serve one contiguous
+// eighth of the cells, so that neighbouring cells share an L2; grids are
+// launched as a multiple of 8 blocks and surplus logical blocks find no cells.
+__device__ __forceinline__ int64_t xcd_block(unsigned b, unsigned grid) {
+  const unsigned per = grid / 8;
+  return int64_t(b % 8) * per + b / 8;
+}
+
+
 // Cross-lane exchange inside a group of NV consecutive lanes of a wavefront
 // (one cell per group; lane g of the group owns conserved variable g).
 struct WaveGroup {
-  int base;  // first lane of the group
+  int base;     // first lane of the group
+  double *slice;  // the group's LDS slice
   __device__ __forceinline__ double bcast(double v, int k) const { return __shfl(v, base + k, 64); }
+  __device__ __forceinline__ int32_t bcast_i(int32_t v, int k) const { return __shfl(v, base + k, 64); }
+  __device__ __forceinline__ double *lds() const { return slice; }
+  // orders the group's LDS writes before its reads (and reads before the
+  // next writes): the lanes of a group share a wavefront, whose LDS
+  // operations execute in order
+  __device__ __forceinline__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 };
 
 // groups of NV lanes per 64-wide wavefront (2D: 16 cells, 3D: 12 cells)
@@ -56,18 +77,21 @@ __global__ __launch_bounds__(256) void k_reconstruct(Img m, const double *__rest
                                                      double *__restrict__ src) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
+  __shared__ double slices[4][GPW][NV * (DOFC - 1)];
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const int grp = lane / NV;
   const int g = lane - grp * NV;
-  const int64_t i = (int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * GPW + grp;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t i = (blk * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= m.n) return;  // whole groups leave together
-  WaveGroup x{lane - g};
+  WaveGroup x{lane - g, &slices[wave][grp][0]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
 }
 
 template <int DIM, int Q>
 __global__ __launch_bounds__(256) void k_face_flux(Img m, const double *__restrict__ fpv, double *__restrict__ G) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   face_flux<DIM, Q>(m, e, fpv, G);
 }
@@ -86,7 +110,7 @@ __global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage
                                                 const double *__restrict__ src, const double *U0,
                                                 double *__restrict__ K, double *Unext) {
   constexpr int NV = DIM + 2;
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int64_t n = m.n;
   if (i >= n) return;
   double L[NV];
@@ -171,6 +195,7 @@ template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
+  __shared__ double slices[4][GPW][NV * (DOFC - 1)];
   const BlockRef b = blk[blockIdx.x];
   if (b.inst < 0) return;
   const DevInst &d = insts[b.inst];
@@ -178,9 +203,10 @@ __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const Bloc
   const int lane = threadIdx.x & 63;
   const int grp = lane / NV;
   const int g = lane - grp * NV;
-  const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * GPW + grp;
+  const int wave = threadIdx.x >> 6;
+  const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= d.m.n) return;
-  WaveGroup x{lane - g};
+  WaveGroup x{lane - g, &slices[wave][grp][0]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
 }
 
@@ -332,27 +358,29 @@ struct Launcher {
 };
 
 inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+// blocks for n items at bs per block, rounded up to a multiple of 8 (xcd_block)
+inline unsigned xcd_grid(int64_t n, int bs) { return (blocks_for(n, bs) + 7u) / 8u * 8u; }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
 Launcher make_launcher() {
   Launcher l;
   l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
     const int cpb = waves * groups_per_wave<DIM + 2>();
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(blocks_for(m.n, cpb)),
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(xcd_grid(m.n, cpb)),
                        dim3(64 * waves), 0, s, m, U, fpv, src);
   };
   l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
     const int cpb = waves * groups_per_wave<DIM + 2>();
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(blocks_for(m.n, cpb)),
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(xcd_grid(m.n, cpb)),
                        dim3(64 * waves), 0, s, m, U, fpv, src);
   };
   l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
     if (m.ne > 0)
-      hipLaunchKernelGGL((k_face_flux<DIM, Q>), dim3(blocks_for(m.ne, 256)), dim3(256), 0, s, m, fpv, G);
+      hipLaunchKernelGGL((k_face_flux<DIM, Q>), dim3(xcd_grid(m.ne, 256)), dim3(256), 0, s, m, fpv, G);
   };
   l.update = [](const Img &m, StageCoeffs sc, int stage, double dt, const double *Us, const double *G,
                 const double *src, const double *U0, double *K, double *Un, hipStream_t s) {
-    hipLaunchKernelGGL((k_update<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, sc, stage, dt, Us, G, src,
+    hipLaunchKernelGGL((k_update<DIM>), dim3(xcd_grid(m.n, 256)), dim3(256), 0, s, m, sc, stage, dt, Us, G, src,
                        U0, K, Un);
   };
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {

@@ -403,8 +403,9 @@ ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f)
 // A group of NV lanes shares one cell: lane g owns conserved variable g (rho,
 // momentum components, energy) and keeps only that variable's polynomial
 // coefficients, so a lane needs a fraction of the registers of a whole cell.
-// The lanes exchange values through X::bcast(v, k) (value v of lane k of the
-// same group):
+// The lanes exchange values through X::bcast(v, k) / X::bcast_i(v, k) (value
+// v of lane k of the same group) and a per-group LDS slice X::lds() of
+// NV x (DOFC - 1) doubles ordered by X::sync():
 //   - the equilibrium averages of the stencil members are computed by lane
 //     (row mod NV) and broadcast to the density and energy lanes,
 //   - the smoothness indicator of a stencil is the maximum over the variables,
@@ -554,13 +555,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 
   const EqC thc = eq_consts<PH>(m, th, phi_ref);
 
-  // equilibrium average of this cell's equilibrium over cell j (shifted)
-  auto member_eq = [&](int64_t j, int code, double &er, double &eE) {
-    double w3[DIM + 1][3];
-    load_corners<DIM>(m, j, w3);
-    for (int k = 0; k < DIM + 1; ++k)
-      for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
-    double vol_j = m.vol[j];
+  // equilibrium average of this cell's equilibrium over a member cell given
+  // its (translated) corners and volume
+  auto member_avg = [&](const double (*w3)[3], double vol_j, double &er, double &eE) {
     double sr = 0.0, sE = 0.0;
     for (int q = 0; q < m.qv; ++q) {
       double x[3];
@@ -580,8 +577,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     eE = (1.0 * sE) / vol_j;
   };
   const bool eq_sub = wb && eq_found;
-  // this cell's equilibrium averaged over itself (member_eq(i, 0) with the
-  // points split over the group's lanes)
+  // this cell's equilibrium averaged over the cell itself, the points split
+  // over the group's lanes
   auto own_eq = [&](double &er, double &eE) {
     const double vol_i = m.vol[i];
     double sr = 0.0, sE = 0.0;
@@ -634,18 +631,56 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
 
   if (!first_order) {
-    // rows (central, then one-sided) in chunks of NV: lane k of the group
-    // computes the equilibrium average of row r0 + k, then every lane
-    // accumulates the chunk's rows.  The chunk loop stays rolled so that only
-    // one chunk's loads are in flight; the one-sided target stencil is picked
-    // with compile-time indices so b stays in registers.
+    // Rows (central, then one-sided) in chunks of NV.  Lane k of the group
+    // owns row r0 + k of a chunk: it loads that row's member (corners and
+    // volume for the equilibrium average) and the row of the pseudo-inverse,
+    // which reaches every lane through the group's LDS slice.  Each chunk
+    // issues all its loads before the member's equilibrium average is
+    // computed, and the next chunk's row entries are loaded one chunk ahead,
+    // so memory latency overlaps arithmetic.  The chunk loop stays rolled.
+    double *pl = xch.lds();  // NV rows x NC coefficients
+    int32_t rw_cur = g < NR ? m.rows[g * n + i] : int32_t(0);
 #pragma unroll 1
     for (int r0 = 0; r0 < NR; r0 += NV) {
-      double my_er = 0.0, my_eE = 0.0;
-      if (eq_sub && r0 + g < NR) {
-        const int32_t rw = m.rows[(r0 + g) * n + i];
-        member_eq(rw & ROW_CELL_MASK, rw >> ROW_CELL_BITS, my_er, my_eE);
+      const int my_r = r0 + g;
+      const bool mine = my_r < NR;
+      // (1) this chunk's loads: the member of lane g, the state of every row
+      // in this lane's variable, lane g's row of P, the next row entry
+      double w3[DIM + 1][3];
+      double vol_j = 1.0;
+      if (eq_sub && mine) {
+        const int64_t j = rw_cur & ROW_CELL_MASK;
+        const int code = rw_cur >> ROW_CELL_BITS;
+        load_corners<DIM>(m, j, w3);
+        for (int k = 0; k < DIM + 1; ++k)
+          for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
+        vol_j = m.vol[j];
       }
+      double uk[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
+      double pr[NC];
+#pragma unroll
+      for (int a = 0; a < NC; ++a) pr[a] = 0.0;
+      if (my_r < RC) {
+#pragma unroll
+        for (int a = 0; a < NC; ++a) pr[a] = m.P[(a * RC + my_r) * n + i];
+      } else if (mine) {
+        const int kb = (my_r - RC) / RB;
+        const int rr = (my_r - RC) - kb * RB;
+        const int off = NC * RC + kb * NBC * RB;
+#pragma unroll
+        for (int a = 0; a < NBC; ++a) pr[a] = m.P[(off + a * RB + rr) * n + i];
+      }
+      const int32_t rw_next = my_r + NV < NR ? m.rows[(my_r + NV) * n + i] : int32_t(0);
+      // (2) equilibrium average of lane g's member
+      double my_er = 0.0, my_eE = 0.0;
+      if (eq_sub && mine) member_avg(w3, vol_j, my_er, my_eE);
+      // (3) the chunk's P rows to every lane, then accumulate its rows
+      xch.sync();
+#pragma unroll
+      for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
+      xch.sync();
 #pragma unroll
       for (int k = 0; k < NV; ++k) {
         double er = 0.0, eE = 0.0;
@@ -655,9 +690,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         }
         const int r = r0 + k;
         if (r < NR) {
-          const int32_t rw = m.rows[r * n + i];
-          const int64_t j = rw & ROW_CELL_MASK;
-          double d = U[g * n + j];
+          double d = uk[k];
           if (wb) {
             if (is_rho) d -= er;
             if (is_E) d -= eE;
@@ -665,14 +698,12 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           d = d * inv_g - q0;
           if (r < RC) {
 #pragma unroll
-            for (int a = 0; a < NC; ++a) c[a] = fma(m.P[(a * RC + r) * n + i], d, c[a]);
+            for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
           } else {
             const int kb = (r - RC) / RB;
-            const int rr = (r - RC) - kb * RB;
-            const int off = NC * RC + kb * NBC * RB;
 #pragma unroll
             for (int a = 0; a < NBC; ++a) {
-              const double p = m.P[(off + a * RB + rr) * n + i];
+              const double p = pl[k * NC + a];
 #pragma unroll
               for (int kk = 0; kk < NB; ++kk)
                 if (kk == kb) b[kk][a] = fma(p, d, b[kk][a]);
@@ -680,6 +711,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           }
         }
       }
+      rw_cur = rw_next;
     }
   }
 
