@@ -421,7 +421,6 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   constexpr int NC = DOFC - 1;
   constexpr int NBC = DIM;
   constexpr int DEGC = deg_of<DIM>(DOFC);
-  constexpr int NR = RC + NB * RB;
   const int64_t n = m.n;
   const double gamma = m.gamma;
 
@@ -558,6 +557,23 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // equilibrium average of this cell's equilibrium over a member cell given
   // its (translated) corners and volume
   auto member_avg = [&](const double (*w3)[3], double vol_j, double &er, double &eE) {
+    if (PH == PH_LIGHT) {
+      // axial gravity: phi is affine, so its point values follow from the
+      // corners; rho_eq = th0 exp(a (phi - phi_ref)) and E_eq = b rho_eq
+      // (b constant), so both averages come from one weighted sum
+      double pv[DIM + 1];
+      for (int k = 0; k < DIM + 1; ++k) pv[k] = phi<DIM, PH>(m, w3[k]);
+      double se = 0.0;
+      for (int q = 0; q < m.qv; ++q) {
+        double ph = m.vx[q][0] * pv[0];
+        for (int k = 1; k < DIM + 1; ++k) ph = ph + m.vx[q][k] * pv[k];
+        const double e = exp((ph - thc.phi_ref) * thc.a);
+        se = (q == 0) ? m.vw[q] * e : se + m.vw[q] * e;
+      }
+      er = thc.th0 * se;
+      eE = er * thc.b;
+      return;
+    }
     double sr = 0.0, sE = 0.0;
     for (int q = 0; q < m.qv; ++q) {
       double x[3];
@@ -631,52 +647,49 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
 
   if (!first_order) {
-    // Rows (central, then one-sided) in chunks of NV.  Lane k of the group
-    // owns row r0 + k of a chunk: it loads that row's member (corners and
-    // volume for the equilibrium average) and the row of the pseudo-inverse,
-    // which reaches every lane through the group's LDS slice.  Each chunk
-    // issues all its loads before the member's equilibrium average is
-    // computed, and the next chunk's row entries are loaded one chunk ahead,
-    // so memory latency overlaps arithmetic.  The chunk loop stays rolled.
+    // Rows in chunks of NV: lane k of the group owns row r0 + k of a chunk.
+    // It loads that row's member (corners and volume for the equilibrium
+    // average) and the row of the pseudo-inverse, which reaches every lane
+    // through the group's LDS slice.  A chunk issues all its loads before the
+    // member's equilibrium average is computed, and the next chunk's row
+    // entries are loaded one chunk ahead, so memory latency overlaps
+    // arithmetic.  Central rows run in a rolled loop; the one-sided stencils
+    // are unrolled so that their coefficients stay in registers.
     double *pl = xch.lds();  // NV rows x NC coefficients
-    int32_t rw_cur = g < NR ? m.rows[g * n + i] : int32_t(0);
-#pragma unroll 1
-    for (int r0 = 0; r0 < NR; r0 += NV) {
-      const int my_r = r0 + g;
-      const bool mine = my_r < NR;
-      // (1) this chunk's loads: the member of lane g, the state of every row
-      // in this lane's variable, lane g's row of P, the next row entry
+    // loads of lane g's member and the equilibrium average over it
+    auto member_of = [&](int32_t rw, double &er, double &eE) {
+      const int64_t j = rw & ROW_CELL_MASK;
+      const int code = rw >> ROW_CELL_BITS;
       double w3[DIM + 1][3];
-      double vol_j = 1.0;
-      if (eq_sub && mine) {
-        const int64_t j = rw_cur & ROW_CELL_MASK;
-        const int code = rw_cur >> ROW_CELL_BITS;
-        load_corners<DIM>(m, j, w3);
-        for (int k = 0; k < DIM + 1; ++k)
-          for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
-        vol_j = m.vol[j];
+      load_corners<DIM>(m, j, w3);
+      for (int k = 0; k < DIM + 1; ++k)
+        for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
+      member_avg(w3, m.vol[j], er, eE);
+    };
+    // scaled deviation of row k of the chunk in this lane's variable
+    auto deviation = [&](double u, double er, double eE) {
+      double d = u;
+      if (wb) {
+        if (is_rho) d -= er;
+        if (is_E) d -= eE;
       }
+      return d * inv_g - q0;
+    };
+
+    int32_t rw_cur = g < RC ? m.rows[g * n + i] : int32_t(0);
+#pragma unroll 1
+    for (int r0 = 0; r0 < RC; r0 += NV) {
+      const int my_r = r0 + g;
+      const bool mine = my_r < RC;
       double uk[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
       double pr[NC];
 #pragma unroll
-      for (int a = 0; a < NC; ++a) pr[a] = 0.0;
-      if (my_r < RC) {
-#pragma unroll
-        for (int a = 0; a < NC; ++a) pr[a] = m.P[(a * RC + my_r) * n + i];
-      } else if (mine) {
-        const int kb = (my_r - RC) / RB;
-        const int rr = (my_r - RC) - kb * RB;
-        const int off = NC * RC + kb * NBC * RB;
-#pragma unroll
-        for (int a = 0; a < NBC; ++a) pr[a] = m.P[(off + a * RB + rr) * n + i];
-      }
-      const int32_t rw_next = my_r + NV < NR ? m.rows[(my_r + NV) * n + i] : int32_t(0);
-      // (2) equilibrium average of lane g's member
+      for (int a = 0; a < NC; ++a) pr[a] = mine ? m.P[(a * RC + my_r) * n + i] : 0.0;
+      const int32_t rw_next = my_r + NV < RC ? m.rows[(my_r + NV) * n + i] : int32_t(0);
       double my_er = 0.0, my_eE = 0.0;
-      if (eq_sub && mine) member_avg(w3, vol_j, my_er, my_eE);
-      // (3) the chunk's P rows to every lane, then accumulate its rows
+      if (eq_sub && mine) member_of(rw_cur, my_er, my_eE);
       xch.sync();
 #pragma unroll
       for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
@@ -688,19 +701,50 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           er = xch.bcast(my_er, k);
           eE = xch.bcast(my_eE, k);
         }
-        const int r = r0 + k;
-        if (r < NR) {
-          double d = uk[k];
-          if (wb) {
-            if (is_rho) d -= er;
-            if (is_E) d -= eE;
-          }
-          d = d * inv_g - q0;
-          if (r < RC) {
+        if (r0 + k < RC) {
+          const double d = deviation(uk[k], er, eE);
 #pragma unroll
-            for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
-          } else {
-            const int kb = (r - RC) / RB;
+          for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
+        }
+      }
+      rw_cur = rw_next;
+    }
+
+    // one-sided stencils: rows RC + kb * RB + rr (a rolled loop; the target
+    // stencil's coefficients are picked with compile-time indices)
+    rw_cur = g < RB ? m.rows[(RC + g) * n + i] : int32_t(0);
+#pragma unroll 1
+    for (int kb = 0; kb < NB; ++kb) {
+      const int roff = RC + kb * RB;
+      const int off = NC * RC + kb * NBC * RB;
+#pragma unroll
+      for (int c0 = 0; c0 < RB; c0 += NV) {
+        const int my_rr = c0 + g;
+        const bool mine = my_rr < RB;
+        double uk[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
+        double pr[NBC];
+#pragma unroll
+        for (int a = 0; a < NBC; ++a) pr[a] = mine ? m.P[(off + a * RB + my_rr) * n + i] : 0.0;
+        int32_t rw_next = 0;
+        if (my_rr + NV < RB) rw_next = m.rows[(roff + my_rr + NV) * n + i];
+        else if (c0 + NV >= RB && kb + 1 < NB && g < RB) rw_next = m.rows[(roff + RB + g) * n + i];
+        double my_er = 0.0, my_eE = 0.0;
+        if (eq_sub && mine) member_of(rw_cur, my_er, my_eE);
+        xch.sync();
+#pragma unroll
+        for (int a = 0; a < NBC; ++a) pl[g * NC + a] = pr[a];
+        xch.sync();
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          double er = 0.0, eE = 0.0;
+          if (wb) {
+            er = xch.bcast(my_er, k);
+            eE = xch.bcast(my_eE, k);
+          }
+          if (c0 + k < RB) {
+            const double d = deviation(uk[k], er, eE);
 #pragma unroll
             for (int a = 0; a < NBC; ++a) {
               const double p = pl[k * NC + a];
@@ -710,8 +754,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
             }
           }
         }
+        rw_cur = rw_next;
       }
-      rw_cur = rw_next;
     }
   }
 
@@ -796,8 +840,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 
   // this variable of the scaled reconstruction at x, given the equilibrium
   // point values (r_eq, E_eq) there; delta is the scaled deviation
+  const double inv_len = 1.0 / len;
   auto eval = [&](const double *x, double r_eq, double E_eq, double &u, double &delta) {
-    double Xs[3] = {(x[0] - xc[0]) / len, (x[1] - xc[1]) / len, DIM == 3 ? (x[2] - xc[2]) / len : 0.0};
+    double Xs[3] = {(x[0] - xc[0]) * inv_len, (x[1] - xc[1]) * inv_len, DIM == 3 ? (x[2] - xc[2]) * inv_len : 0.0};
     double px;
     poly_eval<DIM, DEGC, 1>(reinterpret_cast<const double(*)[1]>(a), cm, Xs, &px);
     delta = scale_g * px;
