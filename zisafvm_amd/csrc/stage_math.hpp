@@ -676,7 +676,25 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       return d * inv_g - q0;
     };
 
+    // translated corners and volume of a member (row entry rw)
+    struct Member {
+      double w3[DIM + 1][3];
+      double vol;
+    };
+    auto load_member = [&](int32_t rw, Member &mb) {
+      const int64_t j = rw & ROW_CELL_MASK;
+      const int code = rw >> ROW_CELL_BITS;
+      load_corners<DIM>(m, j, mb.w3);
+      for (int k = 0; k < DIM + 1; ++k)
+        for (int d = 0; d < DIM; ++d) mb.w3[k][d] += m.shift[code][d];
+      mb.vol = m.vol[j];
+    };
+    // central rows: the row entries run two chunks ahead and lane g's member
+    // one chunk ahead of the chunk being accumulated
     int32_t rw_cur = g < RC ? m.rows[g * n + i] : int32_t(0);
+    int32_t rw_nxt = NV + g < RC ? m.rows[(NV + g) * n + i] : int32_t(0);
+    Member mb_cur{};
+    if (eq_sub && g < RC) load_member(rw_cur, mb_cur);
 #pragma unroll 1
     for (int r0 = 0; r0 < RC; r0 += NV) {
       const int my_r = r0 + g;
@@ -687,9 +705,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double pr[NC];
 #pragma unroll
       for (int a = 0; a < NC; ++a) pr[a] = mine ? m.P[(a * RC + my_r) * n + i] : 0.0;
-      const int32_t rw_next = my_r + NV < RC ? m.rows[(my_r + NV) * n + i] : int32_t(0);
+      Member mb_nxt{};
+      if (eq_sub && my_r + NV < RC) load_member(rw_nxt, mb_nxt);
+      const int32_t rw_far = my_r + 2 * NV < RC ? m.rows[(my_r + 2 * NV) * n + i] : int32_t(0);
       double my_er = 0.0, my_eE = 0.0;
-      if (eq_sub && mine) member_of(rw_cur, my_er, my_eE);
+      if (eq_sub && mine) member_avg(mb_cur.w3, mb_cur.vol, my_er, my_eE);
       xch.sync();
 #pragma unroll
       for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
@@ -707,7 +727,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
         }
       }
-      rw_cur = rw_next;
+      rw_cur = rw_nxt;
+      rw_nxt = rw_far;
+      mb_cur = mb_nxt;
     }
 
     // one-sided stencils: rows RC + kb * RB + rr (a rolled loop; the target
