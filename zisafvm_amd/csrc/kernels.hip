@@ -72,8 +72,10 @@ constexpr int groups_per_wave() {
 }
 
 // k_reconstruct: blocks of `blockDim.x / 64` waves, 64 / NV cells per wave
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
-__global__ __launch_bounds__(256) void k_reconstruct(Img m, const double *__restrict__ U, double *__restrict__ fpv,
+// OCC = 1 asks the compiler for at most 128 VGPRs (4 waves per SIMD), which
+// may spill; OCC = 0 leaves the allocation free.  Both round identically.
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void k_reconstruct(Img m, const double *__restrict__ U, double *__restrict__ fpv,
                                                      double *__restrict__ src) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
@@ -191,8 +193,8 @@ struct BlockRef {
   int local;  // block index inside the instance
 };
 
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
-__global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
   __shared__ double slices[4][GPW][NV * (DOFC - 1)];
@@ -341,15 +343,15 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
 // launch plan for one shape
 
 struct Launcher {
-  // rec[ph]: physics set ph (PH_LIGHT / PH_FULL); rec_waves: waves per
-  // reconstruction block (1, 2 or 4)
-  void (*rec[2])(const Img &, const double *, double *, double *, int rec_waves, hipStream_t);
+  // rec[ph]: physics set ph (PH_LIGHT / PH_FULL); cfg: reconstruction
+  // launch configuration (rec_cfg_waves / rec_cfg_occ)
+  void (*rec[2])(const Img &, const double *, double *, double *, int cfg, hipStream_t);
   void (*face)(const Img &, const double *, double *, hipStream_t);
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
   // batched variants: grid = number of block-table entries
-  void (*brec[2])(DevInst *, const BlockRef *, unsigned, int, int rec_waves, hipStream_t);
+  void (*brec[2])(DevInst *, const BlockRef *, unsigned, int, int cfg, hipStream_t);
   void (*bface)(DevInst *, const BlockRef *, unsigned, hipStream_t);
   void (*bupdate)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
   void (*bcfl)(DevInst *, const BlockRef *, unsigned, hipStream_t);
@@ -361,18 +363,41 @@ inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / b
 // blocks for n items at bs per block, rounded up to a multiple of 8 (xcd_block)
 inline unsigned xcd_grid(int64_t n, int bs) { return (blocks_for(n, bs) + 7u) / 8u * 8u; }
 
+// Reconstruction launch configurations: cfg = 3 * occ + w, waves per block
+// {1, 2, 4}[w], occupancy hint occ (OCC template parameter).
+constexpr int kRecConfigs = 6;
+inline int rec_cfg_waves(int cfg) { return cfg % 3 == 0 ? 1 : (cfg % 3 == 1 ? 2 : 4); }
+inline int rec_cfg_occ(int cfg) { return cfg / 3; }
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
+void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
+  const int waves = rec_cfg_waves(cfg);
+  const int cpb = waves * groups_per_wave<DIM + 2>();
+  if (rec_cfg_occ(cfg))
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 1>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
+                       s, m, U, fpv, src);
+  else
+    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 0>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
+                       s, m, U, fpv, src);
+}
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
+void launch_brec(DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
+  const int waves = rec_cfg_waves(cfg);
+  if (rec_cfg_occ(cfg))
+    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 1>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
+  else
+    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 0>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
+}
+
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
 Launcher make_launcher() {
   Launcher l;
-  l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
-    const int cpb = waves * groups_per_wave<DIM + 2>();
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(xcd_grid(m.n, cpb)),
-                       dim3(64 * waves), 0, s, m, U, fpv, src);
+  l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
+    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(m, U, fpv, src, cfg, s);
   };
-  l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int waves, hipStream_t s) {
-    const int cpb = waves * groups_per_wave<DIM + 2>();
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(xcd_grid(m.n, cpb)),
-                       dim3(64 * waves), 0, s, m, U, fpv, src);
+  l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
+    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(m, U, fpv, src, cfg, s);
   };
   l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
     if (m.ne > 0)
@@ -386,13 +411,11 @@ Launcher make_launcher() {
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
     hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
   };
-  l.brec[PH_LIGHT] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int waves, hipStream_t s) {
-    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>), dim3(nb), dim3(64 * waves), 0, s, d, b,
-                       stage);
+  l.brec[PH_LIGHT] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
+    launch_brec<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(d, b, nb, stage, cfg, s);
   };
-  l.brec[PH_FULL] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int waves, hipStream_t s) {
-    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH_FULL>), dim3(nb), dim3(64 * waves), 0, s, d, b,
-                       stage);
+  l.brec[PH_FULL] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
+    launch_brec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(d, b, nb, stage, cfg, s);
   };
   l.bface = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
     hipLaunchKernelGGL((kb_face_flux<DIM, Q>), dim3(nb), dim3(256), 0, s, d, b);
@@ -449,8 +472,8 @@ struct zfv_handle {
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = -1.0;
   int graph_steps = 0;
-  int rec_waves = 0;  // waves per reconstruction block; 0 = not tuned yet
-  double rec_ms[3] = {0, 0, 0};
+  int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
+  double rec_ms[6] = {0, 0, 0, 0, 0, 0};
   int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL)
 
   template <class T>
@@ -525,24 +548,22 @@ static StageCoeffs ssp3_coeffs(int stage) {
   return c;
 }
 
-// Autotuner of the reconstruction launch geometry: every candidate (1, 2, 4
-// waves per block) is launched on the instance's own state, one warm-up and
-// five timed launches between HIP events on the handle's stream; the fastest
-// average wins.  The launches only write the face-point and source buffers.
-static const int kRecWaves[3] = {1, 2, 4};
-
+// Autotuner of the reconstruction launch configuration: every candidate (1,
+// 2, 4 waves per block, with and without the occupancy hint) is launched on
+// the instance's own state, one warm-up and five timed launches between HIP
+// events on the handle's stream; the fastest average wins.  The launches only
+// write the face-point and source buffers.
 static void tune_rec(zfv_handle *h, const double *Ustate) {
   const Img &m = h->dimg;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
-  int best = 4;
+  int best = kRecConfigs - 1;
   double best_ms = 1e300;
-  for (int c = 0; c < 3; ++c) {
-    const int w = kRecWaves[c];
-    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, w, h->stream);
+  for (int c = 0; c < kRecConfigs; ++c) {
+    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, h->stream);
     HIP_CHECK(hipEventRecord(e0, h->stream));
-    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, w, h->stream);
+    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, h->stream);
     HIP_CHECK(hipEventRecord(e1, h->stream));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipGetLastError());
@@ -551,12 +572,12 @@ static void tune_rec(zfv_handle *h, const double *Ustate) {
     h->rec_ms[c] = ms / 5.0;
     if (h->rec_ms[c] < best_ms) {
       best_ms = h->rec_ms[c];
-      best = w;
+      best = c;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  h->rec_waves = best;
+  h->rec_cfg = best;
 }
 
 // the narrowest physics set whose kernel covers this instance
@@ -564,17 +585,17 @@ static int physics_set(const Img &m) {
   return (m.eq == 0 || m.eq == 2) && (m.gravity == 0 || m.gravity == 1) ? PH_LIGHT : PH_FULL;
 }
 
-// waves per reconstruction block, tuning on first use (never inside a capture)
-static int rec_waves(zfv_handle *h, const double *Ustate) {
-  if (h->rec_waves == 0) tune_rec(h, Ustate);
-  return h->rec_waves;
+// reconstruction launch configuration, tuning on first use (never inside a capture)
+static int rec_config(zfv_handle *h, const double *Ustate) {
+  if (h->rec_cfg < 0) tune_rec(h, Ustate);
+  return h->rec_cfg;
 }
 
 static void enqueue_step(zfv_handle *h, double dt) {
   const Img &m = h->dimg;
   const double *Us = h->U;
   for (int s = 0; s < 3; ++s) {
-    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_waves, h->stream);
+    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, h->stream);
     h->L.face(m, h->fpv, h->G, h->stream);
     double *Un = (s == 2) ? h->U : h->X;
     h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
@@ -604,6 +625,24 @@ static void from_soa(const zfv_handle *h, const double *soa, double *aos, bool a
     for (int d = 0; d < dim; ++d) w[1 + d] = soa[(1 + d) * n + ip];
     for (int v = 0; v < 5; ++v) u[v] = accumulate ? u[v] + w[v] : w[v];
   }
+}
+
+// launch configuration named by (threads per block, occupancy hint)
+static int rec_cfg_of(int32_t rec_threads, int32_t occupancy_hint) {
+  if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
+    throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, -1, 64, 128 or 256");
+  if (occupancy_hint != 0 && occupancy_hint != 4) throw ZfvError(ZFV_ERR_INVALID, "occupancy_hint must be 0 or 4");
+  return (occupancy_hint ? 3 : 0) + (rec_threads == 64 ? 0 : (rec_threads == 128 ? 1 : 2));
+}
+
+static void report_tuning(int cfg, int groups_per_wave, const double *ms, zfv_tuning *out) {
+  if (!out) return;
+  const int c = cfg < 0 ? kRecConfigs - 1 : cfg;
+  out->rec_threads = 64 * rec_cfg_waves(c);
+  out->cells_per_block = rec_cfg_waves(c) * groups_per_wave;
+  out->occupancy_hint = rec_cfg_occ(c) ? 4 : 0;
+  out->tuned = cfg >= 0;
+  for (int k = 0; k < kRecConfigs; ++k) out->rec_ms[k] = ms[k];
 }
 
 }  // namespace zfv
@@ -736,7 +775,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
   to_soa(h, u, h->host_stage);
   HIP_CHECK(hipMemcpyAsync(h->X, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
   const Img &m = h->dimg;
-  h->L.rec[h->ph](m, h->X, h->fpv, h->src, rec_waves(h, h->X), h->stream);
+  h->L.rec[h->ph](m, h->X, h->fpv, h->src, rec_config(h, h->X), h->stream);
   h->L.face(m, h->fpv, h->G, h->stream);
   StageCoeffs sc{};
   sc.write_rate_only = 1;
@@ -752,7 +791,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
 int zfv_step(zfv_handle *h, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
-  rec_waves(h, h->U);
+  rec_config(h, h->U);
   enqueue_step(h, dt);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -764,7 +803,7 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
   if (n_steps < 0) throw ZfvError(ZFV_ERR_INVALID, "n_steps < 0");
-  rec_waves(h, h->U);
+  rec_config(h, h->U);
   if (!h->graph_exec || h->graph_dt != dt) {
     if (h->graph_exec) {
       HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
@@ -836,24 +875,18 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   ZFV_API_END
 }
 
-int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out) {
+int zfv_tune(zfv_handle *h, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out) {
   ZFV_API_BEGIN
   if (rec_threads == 0) {
     tune_rec(h, h->U);
   } else if (rec_threads > 0) {
-    if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
-      throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, 64, 128 or 256");
-    h->rec_waves = rec_threads / 64;
+    h->rec_cfg = rec_cfg_of(rec_threads, occupancy_hint);
   }
-  if (h->graph_exec && rec_threads >= 0) {  // the captured step holds the old geometry
+  if (h->graph_exec && rec_threads >= 0) {  // the captured step holds the old configuration
     HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
     h->graph_exec = nullptr;
   }
-  if (out) {
-    out->rec_threads = 64 * h->rec_waves;
-    out->cells_per_block = h->rec_waves * h->L.groups_per_wave;
-    for (int c = 0; c < 3; ++c) out->rec_ms[c] = h->rec_ms[c];
-  }
+  report_tuning(h->rec_cfg, h->L.groups_per_wave, h->rec_ms, out);
   return ZFV_OK;
   ZFV_API_END
 }
@@ -876,7 +909,7 @@ int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out) {
   ZFV_API_BEGIN
   const Img &m = h->dimg;
-  const int waves = rec_waves(h, h->U);
+  const int cfg = rec_config(h, h->U);
   std::vector<hipEvent_t> ev(2 * 9 * std::max(n_steps, 1));
   for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
   int k = 0;
@@ -884,7 +917,7 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
     const double *Us = h->U;
     for (int s = 0; s < 3; ++s) {
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
-      h->L.rec[h->ph](m, Us, h->fpv, h->src, waves, h->stream);
+      h->L.rec[h->ph](m, Us, h->fpv, h->src, cfg, h->stream);
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       h->L.face(m, h->fpv, h->G, h->stream);
@@ -1007,8 +1040,8 @@ struct zfv_batch {
   int64_t cap_cell_blocks = 0, cap_face_blocks = 0, cap_rec_blocks = 0;
   int64_t n_cell_blocks = 0, n_face_blocks = 0, n_rec_blocks = 0;  // current launch sizes
   bool xcd_chunks = true;
-  int rec_waves = 0;  // waves per reconstruction block; 0 = not tuned yet
-  double rec_ms[3] = {0, 0, 0};
+  int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
+  double rec_ms[6] = {0, 0, 0, 0, 0, 0};
   int ph = 0;         // physics set: the widest any member needs
   zfv::DevInst *d_inst = nullptr;
   zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr, *d_brec = nullptr;
@@ -1081,7 +1114,7 @@ static void batch_upload(zfv_batch *b) {
   b->ph = ph;
   auto tc = block_table(cells, 256, b->cap_cell_blocks, b->xcd_chunks);
   auto tf = block_table(faces, 256, b->cap_face_blocks, b->xcd_chunks);
-  const int waves = b->rec_waves > 0 ? b->rec_waves : 4;
+  const int waves = rec_cfg_waves(b->rec_cfg >= 0 ? b->rec_cfg : kRecConfigs - 1);
   auto tr = block_table(cells, int64_t(waves) * b->L.groups_per_wave, b->cap_rec_blocks, b->xcd_chunks);
   if (int64_t(tc.size()) != b->n_cell_blocks || int64_t(tf.size()) != b->n_face_blocks ||
       int64_t(tr.size()) != b->n_rec_blocks) {
@@ -1110,15 +1143,15 @@ static void batch_tune(zfv_batch *b) {
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
-  int best = 4;
+  int best = kRecConfigs - 1;
   double best_ms = 1e300;
-  for (int c = 0; c < 3; ++c) {
-    b->rec_waves = kRecWaves[c];
+  for (int c = 0; c < kRecConfigs; ++c) {
+    b->rec_cfg = c;
     batch_upload(b);
     const unsigned nr = unsigned(b->n_rec_blocks);
-    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, b->rec_waves, b->stream);
+    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
     HIP_CHECK(hipEventRecord(e0, b->stream));
-    for (int k = 0; k < 3; ++k) b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, b->rec_waves, b->stream);
+    for (int k = 0; k < 3; ++k) b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
     HIP_CHECK(hipEventRecord(e1, b->stream));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipGetLastError());
@@ -1127,12 +1160,12 @@ static void batch_tune(zfv_batch *b) {
     b->rec_ms[c] = ms / 3.0;
     if (b->rec_ms[c] < best_ms) {
       best_ms = b->rec_ms[c];
-      best = kRecWaves[c];
+      best = c;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->rec_waves = best;
+  b->rec_cfg = best;
   batch_upload(b);
 }
 
@@ -1149,7 +1182,7 @@ static void batch_enqueue_step(zfv_batch *b) {
   b->L.bcfl(b->d_inst, b->d_bcell, nc, b->stream);
   hipLaunchKernelGGL(kb_choose_dt, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
   for (int s = 0; s < 3; ++s) {
-    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, s, b->rec_waves, b->stream);
+    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, s, b->rec_cfg, b->stream);
     b->L.bface(b->d_inst, b->d_bface, nf, b->stream);
     b->L.bupdate(b->d_inst, b->d_bcell, nc, s, b->stream);
   }
@@ -1257,9 +1290,9 @@ int zfv_batch_remove(zfv_batch *b, int32_t slot) {
 int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   ZFV_API_BEGIN
   if (b->dirty) batch_upload(b);
-  if (b->rec_waves == 0 && batch_has_active(b)) batch_tune(b);
-  if (b->rec_waves == 0) {  // nothing to run on: any geometry will do
-    b->rec_waves = 4;
+  if (b->rec_cfg < 0 && batch_has_active(b)) batch_tune(b);
+  if (b->rec_cfg < 0) {  // nothing to run on: any configuration will do
+    b->rec_cfg = kRecConfigs - 1;
     batch_upload(b);
   }
   if (!b->graph) {
@@ -1276,26 +1309,20 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   ZFV_API_END
 }
 
-int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out) {
+int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out) {
   ZFV_API_BEGIN
   if (b->dirty) batch_upload(b);
   if (rec_threads == 0) {
     if (batch_has_active(b)) batch_tune(b);
   } else if (rec_threads > 0) {
-    if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
-      throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, 64, 128 or 256");
-    b->rec_waves = rec_threads / 64;
+    b->rec_cfg = rec_cfg_of(rec_threads, occupancy_hint);
     batch_upload(b);
   }
   if (b->graph && rec_threads >= 0) {
     HIP_CHECK(hipGraphExecDestroy(b->graph));
     b->graph = nullptr;
   }
-  if (out) {
-    out->rec_threads = 64 * b->rec_waves;
-    out->cells_per_block = b->rec_waves * b->L.groups_per_wave;
-    for (int c = 0; c < 3; ++c) out->rec_ms[c] = b->rec_ms[c];
-  }
+  report_tuning(b->rec_cfg, b->L.groups_per_wave, b->rec_ms, out);
   return ZFV_OK;
   ZFV_API_END
 }

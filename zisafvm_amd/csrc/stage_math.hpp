@@ -563,12 +563,23 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       // (b constant), so both averages come from one weighted sum
       double pv[DIM + 1];
       for (int k = 0; k < DIM + 1; ++k) pv[k] = phi<DIM, PH>(m, w3[k]);
-      double se = 0.0;
-      for (int q = 0; q < m.qv; ++q) {
+      // two points per iteration: independent exponentials interleave; the
+      // sum still runs in point order
+      auto point = [&](int q) {
         double ph = m.vx[q][0] * pv[0];
         for (int k = 1; k < DIM + 1; ++k) ph = ph + m.vx[q][k] * pv[k];
-        const double e = exp((ph - thc.phi_ref) * thc.a);
-        se = (q == 0) ? m.vw[q] * e : se + m.vw[q] * e;
+        return exp((ph - thc.phi_ref) * thc.a);
+      };
+      double se = 0.0;
+      int q = 0;
+      for (; q + 1 < m.qv; q += 2) {
+        const double e0 = point(q), e1 = point(q + 1);
+        se = (q == 0) ? m.vw[q] * e0 : se + m.vw[q] * e0;
+        se = se + m.vw[q + 1] * e1;
+      }
+      if (q < m.qv) {
+        const double e0 = point(q);
+        se = (q == 0) ? m.vw[q] * e0 : se + m.vw[q] * e0;
       }
       er = thc.th0 * se;
       eE = er * thc.b;
