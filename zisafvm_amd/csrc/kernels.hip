@@ -75,8 +75,8 @@ constexpr int groups_per_wave() {
 // OCC = 1 asks the compiler for at most 128 VGPRs (4 waves per SIMD), which
 // may spill; OCC = 0 leaves the allocation free.  Both round identically.
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, int OCC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void k_reconstruct(Img m, const double *__restrict__ U, double *__restrict__ fpv,
-                                                     double *__restrict__ src) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void k_reconstruct(
+    Img m, const double *__restrict__ U, double *__restrict__ fpv, double *__restrict__ src, int skip_far_ghosts) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
   __shared__ double slices[4][GPW][NV * (DOFC - 1)];
@@ -87,6 +87,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t i = (blk * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= m.n) return;  // whole groups leave together
+  // under the frozen BC a ghost cell away from the interior feeds no flux and
+  // its own tendency is overwritten: nothing to reconstruct
+  if (skip_far_ghosts && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
   WaveGroup x{lane - g, &slices[wave][grp][0]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
 }
@@ -115,6 +118,14 @@ __global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int64_t n = m.n;
   if (i >= n) return;
+  if (!sc.write_rate_only && m.steady != nullptr && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) {
+    // frozen far ghost: its tendency is never used
+    for (int v = 0; v < NV; ++v) {
+      K[(int64_t(stage) * NV + v) * n + i] = 0.0;
+      Unext[v * n + i] = m.steady[v * n + i];
+    }
+    return;
+  }
   double L[NV];
   cell_rate<DIM>(m, i, Ustage, G, src, L);
   for (int v = 0; v < NV; ++v) K[(int64_t(stage) * NV + v) * n + i] = L[v];
@@ -208,6 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1
   const int wave = threadIdx.x >> 6;
   const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= d.m.n) return;
+  if (d.m.steady != nullptr && (d.m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
   WaveGroup x{lane - g, &slices[wave][grp][0]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
 }
@@ -235,6 +247,14 @@ __global__ __launch_bounds__(256) void kb_update(DevInst *insts, const BlockRef 
   const int64_t n = m.n;
   if (i >= n) return;
   const double *Us = stage == 0 ? d.U : d.X;
+  double *Un = stage == 2 ? d.U : d.X;
+  if (m.steady != nullptr && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) {
+    for (int v = 0; v < NV; ++v) {
+      d.K[(int64_t(stage) * NV + v) * n + i] = 0.0;
+      Un[v * n + i] = m.steady[v * n + i];
+    }
+    return;
+  }
   double L[NV];
   cell_rate<DIM>(m, i, Us, d.G, d.src, L);
   // SSP3 next-state coefficients (src/zisa/ode/runge_kutta.cpp:154-165)
@@ -242,7 +262,6 @@ __global__ __launch_bounds__(256) void kb_update(DevInst *insts, const BlockRef 
   const double c1 = stage == 1 ? 0.25 : 1.0 / 6;
   const double c2 = 2.0 / 3;
   const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
-  double *Un = stage == 2 ? d.U : d.X;
   for (int v = 0; v < NV; ++v) {
     d.K[(int64_t(stage) * NV + v) * n + i] = L[v];
     double dudt = 0.0;
@@ -345,7 +364,7 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
 struct Launcher {
   // rec[ph]: physics set ph (PH_LIGHT / PH_FULL); cfg: reconstruction
   // launch configuration (rec_cfg_waves / rec_cfg_occ)
-  void (*rec[2])(const Img &, const double *, double *, double *, int cfg, hipStream_t);
+  void (*rec[2])(const Img &, const double *, double *, double *, int cfg, int skip_far_ghosts, hipStream_t);
   void (*face)(const Img &, const double *, double *, hipStream_t);
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, double *, double *, hipStream_t);
@@ -370,15 +389,15 @@ inline int rec_cfg_waves(int cfg) { return cfg % 3 == 0 ? 1 : (cfg % 3 == 1 ? 2 
 inline int rec_cfg_occ(int cfg) { return cfg / 3; }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
-void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
+void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
   const int cpb = waves * groups_per_wave<DIM + 2>();
   if (rec_cfg_occ(cfg))
     hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 1>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
-                       s, m, U, fpv, src);
+                       s, m, U, fpv, src, skip);
   else
     hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 0>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
-                       s, m, U, fpv, src);
+                       s, m, U, fpv, src, skip);
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
@@ -393,11 +412,11 @@ void launch_brec(DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg,
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
 Launcher make_launcher() {
   Launcher l;
-  l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
-    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(m, U, fpv, src, cfg, s);
+  l.rec[PH_LIGHT] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
+    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(m, U, fpv, src, cfg, skip, s);
   };
-  l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, hipStream_t s) {
-    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(m, U, fpv, src, cfg, s);
+  l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
+    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(m, U, fpv, src, cfg, skip, s);
   };
   l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
     if (m.ne > 0)
@@ -548,6 +567,9 @@ static StageCoeffs ssp3_coeffs(int stage) {
   return c;
 }
 
+// steps run under the frozen BC once a steady state is set
+static int frozen(const zfv_handle *h) { return h->dimg.steady != nullptr ? 1 : 0; }
+
 // Autotuner of the reconstruction launch configuration: every candidate (1,
 // 2, 4 waves per block, with and without the occupancy hint) is launched on
 // the instance's own state, one warm-up and five timed launches between HIP
@@ -561,9 +583,9 @@ static void tune_rec(zfv_handle *h, const double *Ustate) {
   int best = kRecConfigs - 1;
   double best_ms = 1e300;
   for (int c = 0; c < kRecConfigs; ++c) {
-    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, h->stream);
+    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
     HIP_CHECK(hipEventRecord(e0, h->stream));
-    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, h->stream);
+    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
     HIP_CHECK(hipEventRecord(e1, h->stream));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipGetLastError());
@@ -595,7 +617,7 @@ static void enqueue_step(zfv_handle *h, double dt) {
   const Img &m = h->dimg;
   const double *Us = h->U;
   for (int s = 0; s < 3; ++s) {
-    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, h->stream);
+    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, frozen(h), h->stream);
     h->L.face(m, h->fpv, h->G, h->stream);
     double *Un = (s == 2) ? h->U : h->X;
     h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
@@ -775,7 +797,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
   to_soa(h, u, h->host_stage);
   HIP_CHECK(hipMemcpyAsync(h->X, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
   const Img &m = h->dimg;
-  h->L.rec[h->ph](m, h->X, h->fpv, h->src, rec_config(h, h->X), h->stream);
+  h->L.rec[h->ph](m, h->X, h->fpv, h->src, rec_config(h, h->X), 0, h->stream);
   h->L.face(m, h->fpv, h->G, h->stream);
   StageCoeffs sc{};
   sc.write_rate_only = 1;
@@ -917,7 +939,7 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
     const double *Us = h->U;
     for (int s = 0; s < 3; ++s) {
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
-      h->L.rec[h->ph](m, Us, h->fpv, h->src, cfg, h->stream);
+      h->L.rec[h->ph](m, Us, h->fpv, h->src, cfg, frozen(h), h->stream);
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       h->L.face(m, h->fpv, h->G, h->stream);

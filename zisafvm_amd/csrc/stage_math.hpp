@@ -32,6 +32,10 @@
 
 namespace zfv {
 
+// cell meta word: bits 0-2 index of the CWENO high-order stencil, 8 = first
+// order (a ghost cell away from the interior), 16 = ghost cell
+constexpr int32_t META_FAR_GHOST = 8 | 16;
+
 constexpr int ROW_CELL_BITS = 27;
 constexpr int32_t ROW_CELL_MASK = (1 << ROW_CELL_BITS) - 1;
 
@@ -174,13 +178,22 @@ ZFV_HD EqC eq_consts(const Img &m, const double *th, double phi_ref) {
   return c;
 }
 
+// x^y with the exponents an ideal gas of gamma = 2 (the n = 1 polytrope)
+// produces, 1 and 2, done exactly (x^1 = x; x^2 is one correctly rounded
+// product); any other exponent goes to pow
+ZFV_HD double ipow(double x, double y) {
+  if (y == 1.0) return x;
+  if (y == 2.0) return x * x;
+  return pow(x, y);
+}
+
 template <int DIM, int PH>
 ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, double &E) {
   const double ph = phi<DIM, PH>(m, x);
   if (PH == PH_FULL && m.eq == 1) {
     double h = c.th0 + c.phi_ref - ph;
-    rho = pow(c.a * h, c.b);
-    double p = c.th1 * pow(rho, m.gamma);
+    rho = ipow(c.a * h, c.b);
+    double p = c.th1 * ipow(rho, m.gamma);
     E = p / (m.gamma - 1.0);
   } else {
     rho = c.th0 * exp((ph - c.phi_ref) * c.a);
@@ -492,7 +505,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       // quasi-Newton with a central finite-difference Jacobian
       // (include/zisa/math/quasi_newton.hpp:12-50)
       double p = Eint0 * (gamma - 1.0);
-      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / pow(rho0, gamma)};
+      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / ipow(rho0, gamma)};
       double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
       double x0 = guess[0], x1 = guess[1];
       auto f = [&](double a, double b, double &f0, double &f1) {
