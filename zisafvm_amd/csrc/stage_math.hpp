@@ -452,6 +452,31 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   double vtx[DIM + 1][3];
   load_corners<DIM>(m, i, vtx);
 
+  // Average of an equilibrium over this cell with the quadrature points split
+  // over the group's lanes: lane k sums the points q = k (mod NV) in order,
+  // then every lane adds the NV partial sums in lane order, so all lanes hold
+  // the same value.
+  auto group_avg = [&](const EqC &tc, double &ar, double &aE) {
+    const double vol_i = m.vol[i];
+    double pr = 0.0, pE = 0.0;
+    for (int q = g; q < m.qv; q += NV) {
+      double x[3];
+      volume_point<DIM>(m, q, vtx, x);
+      double r, E;
+      eq_point<DIM, PH>(m, tc, x, r, E);
+      const double w = vol_i * m.vw[q];
+      pr = pr + w * r;
+      pE = pE + w * E;
+    }
+    double sr = xch.bcast(pr, 0), sE = xch.bcast(pE, 0);
+    for (int k = 1; k < NV; ++k) {
+      sr = sr + xch.bcast(pr, k);
+      sE = sE + xch.bcast(pE, k);
+    }
+    ar = (1.0 * sr) / vol_i;
+    aE = (1.0 * sE) / vol_i;
+  };
+
   // local equilibrium of this cell: every lane solves it (identical results)
   double th[2] = {0.0, 0.0};
   double phi_ref = 0.0;
@@ -462,36 +487,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     volume_point<DIM>(m, 0, vtx, xref);
     phi_ref = phi<DIM, PH>(m, xref);
     double vol_i = m.vol[i];
-    // cell average of the equilibrium with parameters t over this cell: lane
-    // k evaluates the points q = k (mod NV), the sum runs in point order in
-    // every lane
+    // cell average of the equilibrium with parameters t over this cell
     auto cell_avg = [&](const double *t, double &ar, double &aE) {
-      const EqC tc = eq_consts<PH>(m, t, phi_ref);
-      double sr = 0.0, sE = 0.0;
-      for (int q0 = 0; q0 < m.qv; q0 += NV) {
-        double my_r = 0.0, my_E = 0.0;
-        if (q0 + g < m.qv) {
-          double x[3];
-          volume_point<DIM>(m, q0 + g, vtx, x);
-          eq_point<DIM, PH>(m, tc, x, my_r, my_E);
-        }
-        for (int k = 0; k < NV; ++k) {
-          const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
-          const int q = q0 + k;
-          if (q < m.qv) {
-            double w = vol_i * m.vw[q];
-            if (q == 0) {
-              sr = w * r;
-              sE = w * E;
-            } else {
-              sr = sr + w * r;
-              sE = sE + w * E;
-            }
-          }
-        }
-      }
-      ar = (1.0 * sr) / vol_i;
-      aE = (1.0 * sE) / vol_i;
+      group_avg(eq_consts<PH>(m, t, phi_ref), ar, aE);
     };
     if (PH == PH_LIGHT || m.eq == 2) {
       double RT = (gamma - 1.0) * Eint0 / rho0;
@@ -619,35 +617,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   const bool eq_sub = wb && eq_found;
   // this cell's equilibrium averaged over the cell itself, the points split
   // over the group's lanes
-  auto own_eq = [&](double &er, double &eE) {
-    const double vol_i = m.vol[i];
-    double sr = 0.0, sE = 0.0;
-    for (int q0 = 0; q0 < m.qv; q0 += NV) {
-      double my_r = 0.0, my_E = 0.0;
-      if (q0 + g < m.qv) {
-        double x[3];
-        volume_point<DIM>(m, q0 + g, vtx, x);
-        eq_point<DIM, PH>(m, thc, x, my_r, my_E);
-      }
-      for (int k = 0; k < NV; ++k) {
-        const double r = xch.bcast(my_r, k), E = xch.bcast(my_E, k);
-        const int q = q0 + k;
-        if (q < m.qv) {
-          double w = vol_i * m.vw[q];
-          if (q == 0) {
-            sr = w * r;
-            sE = w * E;
-          } else {
-            sr = sr + w * r;
-            sE = sE + w * E;
-          }
-        }
-      }
-    }
-    er = (1.0 * sr) / vol_i;
-    eE = (1.0 * sE) / vol_i;
-  };
-
+  auto own_eq = [&](double &er, double &eE) { group_avg(thc, er, eE); };
   double q0;
   {
     double er = 0.0, eE = 0.0;
