@@ -28,7 +28,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src;
+  std::vector<double> U, fpv, G, src, phim;
 };
 
 // The lanes of one cell's group run as NV host threads in lockstep: every
@@ -162,6 +162,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.cface = im.cface.data();
     m.fperm = im.fperm.data();
     m.steady = nullptr;
+    m.phim = nullptr;
     m.gamma = p->gamma;
     m.eps = p->epsilon;
     m.exponent = p->exponent;
@@ -187,6 +188,16 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     for (size_t q = 0; q < im.vol_w.size(); ++q) {
       m.vw[q] = im.vol_w[q];
       for (int d = 0; d < im.s.dim + 1; ++d) m.vx[q][d] = im.vol_x[q * (im.s.dim + 1) + d];
+    }
+    if (p->equilibrium == ZFV_EQ_ISENTROPIC && p->gravity != ZFV_GRAVITY_NONE && p->gamma == 2.0) {
+      e->phim.assign(size_t(2) * im.n, 0.0);
+      for (int64_t i = 0; i < im.n; ++i) {
+        if (im.s.dim == 2)
+          phi_moments<2>(m, i, e->phim[i], e->phim[im.n + i]);
+        else
+          phi_moments<3>(m, i, e->phim[i], e->phim[im.n + i]);
+      }
+      m.phim = e->phim.data();
     }
     const int nv = im.s.nvars;
     e->fpv.assign(size_t(2) * im.s.q * nv * std::max<int64_t>(im.ne, 1), 0.0);

@@ -181,6 +181,17 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// per-cell potential moments for the linear isentropic equilibrium averages
+template <int DIM>
+__global__ __launch_bounds__(256) void k_phi_moments(Img m, double *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  double mean, var;
+  phi_moments<DIM>(m, i, mean, var);
+  out[i] = mean;
+  out[m.n + i] = var;
+}
+
 // ---------------------------------------------------------------------------
 // Batched execution: many instances of one kernel shape advance together.
 // Each instance keeps its own arrays; a device table of DevInst describes
@@ -703,6 +714,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.cface = h->upload(im.cface);
   m.fperm = h->upload(im.fperm);
   m.steady = nullptr;
+  m.phim = nullptr;
   m.gamma = params->gamma;
   m.eps = params->epsilon;
   m.exponent = params->exponent;
@@ -741,6 +753,16 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   h->G = h->alloc<double>(size_t(nv) * std::max<int64_t>(im.ne, 1));
   if (params->gravity != ZFV_GRAVITY_NONE) h->src = h->alloc<double>(size_t(nv) * n);
   h->dt_bits = h->alloc<unsigned long long>(1);
+  if (params->equilibrium == ZFV_EQ_ISENTROPIC && params->gravity != ZFV_GRAVITY_NONE && params->gamma == 2.0) {
+    double *pm = h->alloc<double>(size_t(2) * n);
+    if (s.dim == 2)
+      hipLaunchKernelGGL(k_phi_moments<2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, pm);
+    else
+      hipLaunchKernelGGL(k_phi_moments<3>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, pm);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    m.phim = pm;
+  }
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
   *out = h.release();
   return ZFV_OK;

@@ -56,6 +56,7 @@ struct Img {
   const int32_t *cface;
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
+  const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   // physics
   double gamma;
   double eps, exponent;
@@ -201,6 +202,20 @@ ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, d
   }
 }
 
+// The isentropic equilibrium of an ideal gas with gamma = 2 is linear in the
+// potential (rho = (h0 - phi) / (2K), E = K rho^2), so its average over a cell
+// needs only the cell's quadrature mean of phi and of its squared deviation:
+//   rho_avg = a (A - phim),  E_avg = th1 a^2 ((A - phim)^2 + phiv)
+// with a = 1/(2K), A = th0 + phi_ref.  Exact in real arithmetic for the same
+// quadrature rule; moments are per cell and time independent.
+ZFV_HD bool linear_isentropic(const Img &m) { return m.eq == 1 && m.gamma == 2.0 && m.phim != nullptr; }
+
+ZFV_HD void linear_isentropic_avg(const EqC &c, double phim, double phiv, double &ar, double &aE) {
+  const double h = c.th0 + c.phi_ref - phim;
+  ar = c.a * h;
+  aE = c.th1 * (c.a * c.a) * (h * h + phiv);
+}
+
 // ---- geometry helpers ---------------------------------------------------------
 template <int DIM>
 ZFV_HD void load_corners(const Img &m, int64_t i, double (*v)[3]) {
@@ -283,6 +298,30 @@ ZFV_HD void volume_point(const Img &m, int q, const double (*v)[3], double *x) {
     x[d] = s;
   }
   if (DIM == 2) x[2] = 0.0;
+}
+
+// Quadrature mean of the potential over cell i and mean squared deviation
+// from it (for linear_isentropic_avg)
+template <int DIM>
+ZFV_HD void phi_moments(const Img &m, int64_t i, double &mean, double &var) {
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+  double s = 0.0;
+  for (int q = 0; q < m.qv; ++q) {
+    double x[3];
+    volume_point<DIM>(m, q, vtx, x);
+    const double p = phi<DIM, PH_FULL>(m, x);
+    s = (q == 0) ? m.vw[q] * p : s + m.vw[q] * p;
+  }
+  double v = 0.0;
+  for (int q = 0; q < m.qv; ++q) {
+    double x[3];
+    volume_point<DIM>(m, q, vtx, x);
+    const double d = phi<DIM, PH_FULL>(m, x) - s;
+    v = (q == 0) ? m.vw[q] * (d * d) : v + m.vw[q] * (d * d);
+  }
+  mean = s;
+  var = v;
 }
 
 // ---- polynomial evaluation (include/zisa/math/poly2d_impl.hpp:271-323) -------
@@ -457,6 +496,10 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // then every lane adds the NV partial sums in lane order, so all lanes hold
   // the same value.
   auto group_avg = [&](const EqC &tc, double &ar, double &aE) {
+    if (PH == PH_FULL && linear_isentropic(m)) {
+      linear_isentropic_avg(tc, m.phim[i], m.phim[n + i], ar, aE);
+      return;
+    }
     const double vol_i = m.vol[i];
     double pr = 0.0, pE = 0.0;
     for (int q = g; q < m.qv; q += NV) {
@@ -650,16 +693,6 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // arithmetic.  Central rows run in a rolled loop; the one-sided stencils
     // are unrolled so that their coefficients stay in registers.
     double *pl = xch.lds();  // NV rows x NC coefficients
-    // loads of lane g's member and the equilibrium average over it
-    auto member_of = [&](int32_t rw, double &er, double &eE) {
-      const int64_t j = rw & ROW_CELL_MASK;
-      const int code = rw >> ROW_CELL_BITS;
-      double w3[DIM + 1][3];
-      load_corners<DIM>(m, j, w3);
-      for (int k = 0; k < DIM + 1; ++k)
-        for (int d = 0; d < DIM; ++d) w3[k][d] += m.shift[code][d];
-      member_avg(w3, m.vol[j], er, eE);
-    };
     // scaled deviation of row k of the chunk in this lane's variable
     auto deviation = [&](double u, double er, double eE) {
       double d = u;
@@ -671,17 +704,32 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     };
 
     // translated corners and volume of a member (row entry rw)
+    // what the equilibrium average over a member needs: its potential
+    // moments (linear isentropic equilibrium, untranslated member) or its
+    // translated corners and volume
     struct Member {
       double w3[DIM + 1][3];
       double vol;
+      double pm, pv;
+      int lin;
     };
     auto load_member = [&](int32_t rw, Member &mb) {
       const int64_t j = rw & ROW_CELL_MASK;
       const int code = rw >> ROW_CELL_BITS;
+      mb.lin = (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
+      if (mb.lin) {
+        mb.pm = m.phim[j];
+        mb.pv = m.phim[n + j];
+        return;
+      }
       load_corners<DIM>(m, j, mb.w3);
       for (int k = 0; k < DIM + 1; ++k)
         for (int d = 0; d < DIM; ++d) mb.w3[k][d] += m.shift[code][d];
       mb.vol = m.vol[j];
+    };
+    auto member_avg_of = [&](const Member &mb, double &er, double &eE) {
+      if (PH == PH_FULL && mb.lin) linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
+      else member_avg(mb.w3, mb.vol, er, eE);
     };
     // central rows: the row entries run two chunks ahead and lane g's member
     // one chunk ahead of the chunk being accumulated
@@ -703,7 +751,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (eq_sub && my_r + NV < RC) load_member(rw_nxt, mb_nxt);
       const int32_t rw_far = my_r + 2 * NV < RC ? m.rows[(my_r + 2 * NV) * n + i] : int32_t(0);
       double my_er = 0.0, my_eE = 0.0;
-      if (eq_sub && mine) member_avg(mb_cur.w3, mb_cur.vol, my_er, my_eE);
+      if (eq_sub && mine) member_avg_of(mb_cur, my_er, my_eE);
       xch.sync();
 #pragma unroll
       for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
@@ -747,7 +795,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         if (my_rr + NV < RB) rw_next = m.rows[(roff + my_rr + NV) * n + i];
         else if (c0 + NV >= RB && kb + 1 < NB && g < RB) rw_next = m.rows[(roff + RB + g) * n + i];
         double my_er = 0.0, my_eE = 0.0;
-        if (eq_sub && mine) member_of(rw_cur, my_er, my_eE);
+        if (eq_sub && mine) {
+          Member mb{};
+          load_member(rw_cur, mb);
+          member_avg_of(mb, my_er, my_eE);
+        }
         xch.sync();
 #pragma unroll
         for (int a = 0; a < NBC; ++a) pl[g * NC + a] = pr[a];
