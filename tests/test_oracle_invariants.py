@@ -230,3 +230,32 @@ def test_well_balanced_reconstruction_small_perturbation():
             x = sum(l * c[k] for k, l in enumerate(lam))
             worst = max(worst, np.linalg.norm(o.eval(int(i), x) - W.polytrope_n1(x[None, :])[0]))
     assert worst < 3.3e-8, worst
+
+
+def test_vortex_convergence_fixture():
+    """The convergence fixture is the oracle's: the coarsest order-3 run
+    recomputed here gives the stored error, and the stored observed orders
+    of the full solve are those of a third- and a fourth-order scheme."""
+    import json
+    import os
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "vortex_convergence.json")) as f:
+        fx = json.load(f)
+    c3 = next(c for c in fx["cases"] if c["order"] == 3)
+    run = c3["runs"][0]
+    mesh = z.lattice_mesh_2d(run["n"], jitter=fx["jitter"], seed=fx["seed"], periodic=(True, True))
+    p = z.make_params(n_dims=2, order=3, flux=z.ZFV_FLUX_HLLC)
+    u = W.cell_average(mesh, lambda x: W.isentropic_vortex(x, beta=fx["beta"]))
+    exact = W.cell_average(mesh, lambda x: W.isentropic_vortex((x - fx["t_end"]) % 1.0, beta=fx["beta"]))
+    o = Oracle(mesh, p, threads=4)
+    t = 0.0
+    for k in range(run["n_steps"]):
+        dt = run["dt"] if k < run["n_steps"] - 1 else run["dt_last"]
+        u = o.step(u, t, dt)
+        t += dt
+    e = float(np.sum(mesh.volumes() * np.abs(u[:, 0] - exact[:, 0])))
+    assert abs(e - float(run["l1_rho"])) <= 1e-12 * e
+    assert 2.8 <= c3["observed_orders"][-1] <= 3.35
+    c4 = next(c for c in fx["cases"] if c["order"] == 4)
+    assert c4["observed_orders"][-1] >= 3.5
