@@ -246,7 +246,10 @@ def main():
     rec_ms = kt.reconstruct_ms / launches
     face_ms = kt.face_flux_ms / launches
     upd_ms = kt.update_ms / launches
-    achieved = info.recon_bytes_per_cell * n / (rec_ms * 1e-3) / 1e9
+    # cells the reconstruction kernel processes: under the frozen BC (a steady
+    # state is set) ghost cells with no interior neighbour are skipped
+    n_rec = n - info.n_far_ghost_cells if wl.steady is not None else n
+    achieved = info.recon_bytes_per_cell * n_rec / (rec_ms * 1e-3) / 1e9
     log(f"kernel avg ms: reconstruct {rec_ms:.4f} face {face_ms:.4f} update {upd_ms:.4f}")
 
     out = {
@@ -266,6 +269,7 @@ def main():
             "workload": wl.name,
             "n_cells": n,
             "n_interior_cells": int((wl.mesh.ghost == 0).sum()),
+            "n_reconstructed_cells": n_rec,
             "order": wl.meta.get("order"),
             "mesh": wl.meta,
             "reconstruction": "CWENO-AO",
@@ -282,6 +286,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(wl.name, n),
+            "cells_per_launch": n_rec,
             "kernel": "k_reconstruct",
             "bytes_per_cell": info.recon_bytes_per_cell,
             "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE of this workload, "

@@ -139,6 +139,9 @@ typedef struct zfv_info {
   double recon_bytes_per_cell;/* algorithmic bytes of the reconstruction kernel */
   int32_t max_stencil_size;   /* largest combined stencil */
   int32_t n_points_face;      /* quadrature points per face */
+  int64_t n_far_ghost_cells;  /* ghost cells with no interior neighbour: once a
+                                 steady state is set, steps leave them frozen
+                                 and do not reconstruct them */
 } zfv_info;
 int zfv_get_info(zfv_handle *h, zfv_info *info);
 
@@ -161,21 +164,19 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
 
 /* Launch configuration of the reconstruction kernel.  No counterpart in the
  * reference.  A cell is served by a group of nvars lanes, 64 / nvars cells per
- * wavefront; a block holds 1, 2 or 4 wavefronts (64, 128, 256 threads), and
- * the kernel is built twice: with a free register allocation and with an
- * occupancy hint of 4 waves per SIMD (at most 128 VGPRs, possibly spilling).
- * The first rate or step call tunes it on the instance's own data by timing
- * all six candidates with HIP events; results never depend on it.  zfv_tune
+ * wavefront; a block holds 1, 2 or 4 wavefronts (64, 128, 256 threads).  The
+ * first rate or step call tunes it on the instance's own data by timing the
+ * three candidates with HIP events; results never depend on it.  zfv_tune
  * re-tunes now (rec_threads = 0), fixes the configuration (rec_threads 64,
- * 128 or 256 with occupancy_hint 0 or 4) or only reports it (-1). */
+ * 128 or 256) or only reports it (-1). */
 typedef struct zfv_tuning {
   int32_t rec_threads;      /* threads per reconstruction block in use */
   int32_t cells_per_block;  /* cells per reconstruction block in use */
-  int32_t occupancy_hint;   /* 0 or 4 waves per SIMD */
   int32_t tuned;            /* 1 once a configuration is chosen */
-  double rec_ms[6];         /* average ms per launch: [hint 0: 64, 128, 256 threads, hint 4: 64, 128, 256] */
+  int32_t reserved;
+  double rec_ms[3];         /* average ms per launch at 64, 128, 256 threads */
 } zfv_tuning;
-int zfv_tune(zfv_handle *h, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out);
+int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out);
 
 /* ---- Batches: many instances of one kernel shape advanced together.
  * Has no counterpart in the reference (which runs one experiment per
@@ -203,7 +204,7 @@ int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *
  * cells so stencil gathers share one L2; 0 is the hardware round robin. */
 /* Reconstruction launch geometry of a batch, as zfv_tune; tuned on the
  * members present at the first zfv_batch_run. */
-int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out);
+int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out);
 int zfv_batch_set_xcd_chunks(zfv_batch *b, int32_t on);
 
 /* Device bytes zfv_create would allocate for this mesh (host-only estimate

@@ -95,11 +95,13 @@ void run(Emu &e, const double *U, double *L) {
         try {
           EmuXch x{&bar, g};
           // the physics set the GPU picks for these parameters
-          const bool light = (e.m.eq == 0 || e.m.eq == 2) && (e.m.gravity == 0 || e.m.gravity == 1);
+          const int ph = physics_set(e.m);
           double *src = e.src.empty() ? nullptr : e.src.data();
           for (int64_t i = 0; i < n; ++i) {
-            if (light)
+            if (ph == PH_LIGHT)
               reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(e.m, i, g, x, U, e.fpv.data(), src);
+            else if (ph == PH_LIN)
+              reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH_LIN>(e.m, i, g, x, U, e.fpv.data(), src);
             else
               reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(e.m, i, g, x, U, e.fpv.data(), src);
           }

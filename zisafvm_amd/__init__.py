@@ -84,6 +84,7 @@ class Info(ctypes.Structure):
         ("recon_bytes_per_cell", ctypes.c_double),
         ("max_stencil_size", ctypes.c_int32),
         ("n_points_face", ctypes.c_int32),
+        ("n_far_ghost_cells", ctypes.c_int64),
     ]
 
 
@@ -91,16 +92,14 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("rec_threads", ctypes.c_int32),
         ("cells_per_block", ctypes.c_int32),
-        ("occupancy_hint", ctypes.c_int32),
         ("tuned", ctypes.c_int32),
-        ("rec_ms", ctypes.c_double * 6),
+        ("reserved", ctypes.c_int32),
+        ("rec_ms", ctypes.c_double * 3),
     ]
 
     def as_dict(self):
         return {"rec_threads": self.rec_threads, "cells_per_block": self.cells_per_block,
-                "occupancy_hint": self.occupancy_hint,
-                "rec_ms_free_regs_64_128_256": list(self.rec_ms[:3]),
-                "rec_ms_hint4_64_128_256": list(self.rec_ms[3:])}
+                "rec_ms_at_64_128_256_threads": list(self.rec_ms)}
 
 
 class KernelTimes(ctypes.Structure):
@@ -217,8 +216,8 @@ def load_library(path: str = LIB_PATH):
         "zfv_batch_run": (c_int, [vp, ctypes.c_int32]),
         "zfv_batch_query": (c_int, [vp, ctypes.c_int32, dp, dp, ip, ip]),
         "zfv_batch_set_xcd_chunks": (c_int, [vp, ctypes.c_int32]),
-        "zfv_tune": (c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Tuning)]),
-        "zfv_batch_tune": (c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Tuning)]),
+        "zfv_tune": (c_int, [vp, ctypes.c_int32, ctypes.POINTER(Tuning)]),
+        "zfv_batch_tune": (c_int, [vp, ctypes.c_int32, ctypes.POINTER(Tuning)]),
         "zfv_mesh_lattice2d": (
             c_int,
             [ctypes.c_int32, c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -401,11 +400,11 @@ class Solver:
         _check(self.lib.zfv_time_kernels(self.h, n_steps, dt, ctypes.byref(kt)))
         return kt
 
-    def tune(self, rec_threads: int = 0, occupancy_hint: int = 0) -> Tuning:
-        """Re-tune (0), report (-1) or fix (64/128/256 threads, hint 0 or 4)
+    def tune(self, rec_threads: int = 0) -> Tuning:
+        """Re-tune (0), report (-1) or fix (64/128/256 threads per block)
         the reconstruction launch configuration."""
         t = Tuning()
-        _check(self.lib.zfv_tune(self.h, rec_threads, occupancy_hint, ctypes.byref(t)))
+        _check(self.lib.zfv_tune(self.h, rec_threads, ctypes.byref(t)))
         return t
 
 
@@ -454,7 +453,7 @@ class Batch:
     def set_xcd_chunks(self, on: bool):
         _check(self.lib.zfv_batch_set_xcd_chunks(self.b, int(on)))
 
-    def tune(self, rec_threads: int = 0, occupancy_hint: int = 0) -> Tuning:
+    def tune(self, rec_threads: int = 0) -> Tuning:
         t = Tuning()
-        _check(self.lib.zfv_batch_tune(self.b, rec_threads, occupancy_hint, ctypes.byref(t)))
+        _check(self.lib.zfv_batch_tune(self.b, rec_threads, ctypes.byref(t)))
         return t

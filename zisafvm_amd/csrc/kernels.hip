@@ -72,10 +72,8 @@ constexpr int groups_per_wave() {
 }
 
 // k_reconstruct: blocks of `blockDim.x / 64` waves, 64 / NV cells per wave
-// OCC = 1 asks the compiler for at most 128 VGPRs (4 waves per SIMD), which
-// may spill; OCC = 0 leaves the allocation free.  Both round identically.
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, int OCC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void k_reconstruct(
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
+__global__ __launch_bounds__(256) void k_reconstruct(
     Img m, const double *__restrict__ U, double *__restrict__ fpv, double *__restrict__ src, int skip_far_ghosts) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
@@ -215,8 +213,8 @@ struct BlockRef {
   int local;  // block index inside the instance
 };
 
-template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, int OCC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1))) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
+__global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
   __shared__ double slices[4][GPW][NV * (DOFC - 1)];
@@ -373,15 +371,15 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
 // launch plan for one shape
 
 struct Launcher {
-  // rec[ph]: physics set ph (PH_LIGHT / PH_FULL); cfg: reconstruction
-  // launch configuration (rec_cfg_waves / rec_cfg_occ)
-  void (*rec[2])(const Img &, const double *, double *, double *, int cfg, int skip_far_ghosts, hipStream_t);
+  // rec[ph]: physics set ph (PH_LIGHT / PH_FULL / PH_LIN); cfg: reconstruction
+  // launch configuration (rec_cfg_waves)
+  void (*rec[3])(const Img &, const double *, double *, double *, int cfg, int skip_far_ghosts, hipStream_t);
   void (*face)(const Img &, const double *, double *, hipStream_t);
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
   // batched variants: grid = number of block-table entries
-  void (*brec[2])(DevInst *, const BlockRef *, unsigned, int, int cfg, hipStream_t);
+  void (*brec[3])(DevInst *, const BlockRef *, unsigned, int, int cfg, hipStream_t);
   void (*bface)(DevInst *, const BlockRef *, unsigned, hipStream_t);
   void (*bupdate)(DevInst *, const BlockRef *, unsigned, int, hipStream_t);
   void (*bcfl)(DevInst *, const BlockRef *, unsigned, hipStream_t);
@@ -393,31 +391,24 @@ inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / b
 // blocks for n items at bs per block, rounded up to a multiple of 8 (xcd_block)
 inline unsigned xcd_grid(int64_t n, int bs) { return (blocks_for(n, bs) + 7u) / 8u * 8u; }
 
-// Reconstruction launch configurations: cfg = 3 * occ + w, waves per block
-// {1, 2, 4}[w], occupancy hint occ (OCC template parameter).
-constexpr int kRecConfigs = 6;
-inline int rec_cfg_waves(int cfg) { return cfg % 3 == 0 ? 1 : (cfg % 3 == 1 ? 2 : 4); }
-inline int rec_cfg_occ(int cfg) { return cfg / 3; }
+// Reconstruction launch configurations: cfg = 0, 1, 2 runs 1, 2, 4 waves per
+// block.  (Variants compiled for 4 waves per SIMD, i.e. at most 128 VGPRs,
+// spilled to scratch and were never faster; they are not built.)
+constexpr int kRecConfigs = 3;
+inline int rec_cfg_waves(int cfg) { return 1 << cfg; }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
   const int cpb = waves * groups_per_wave<DIM + 2>();
-  if (rec_cfg_occ(cfg))
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 1>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
-                       s, m, U, fpv, src, skip);
-  else
-    hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 0>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0,
-                       s, m, U, fpv, src, skip);
+  hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0, s,
+                     m, U, fpv, src, skip);
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_brec(DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
-  if (rec_cfg_occ(cfg))
-    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 1>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
-  else
-    hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH, 0>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
+  hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
@@ -428,6 +419,9 @@ Launcher make_launcher() {
   };
   l.rec[PH_FULL] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
     launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(m, U, fpv, src, cfg, skip, s);
+  };
+  l.rec[PH_LIN] = [](const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
+    launch_rec<DIM, DOFC, RC, NB, RB, Q, PH_LIN>(m, U, fpv, src, cfg, skip, s);
   };
   l.face = [](const Img &m, const double *fpv, double *G, hipStream_t s) {
     if (m.ne > 0)
@@ -446,6 +440,9 @@ Launcher make_launcher() {
   };
   l.brec[PH_FULL] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
     launch_brec<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(d, b, nb, stage, cfg, s);
+  };
+  l.brec[PH_LIN] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
+    launch_brec<DIM, DOFC, RC, NB, RB, Q, PH_LIN>(d, b, nb, stage, cfg, s);
   };
   l.bface = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
     hipLaunchKernelGGL((kb_face_flux<DIM, Q>), dim3(nb), dim3(256), 0, s, d, b);
@@ -503,8 +500,8 @@ struct zfv_handle {
   double graph_dt = -1.0;
   int graph_steps = 0;
   int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
-  double rec_ms[6] = {0, 0, 0, 0, 0, 0};
-  int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL)
+  double rec_ms[zfv::kRecConfigs] = {0, 0, 0};
+  int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL / PH_LIN)
 
   template <class T>
   T *alloc(size_t count) {
@@ -582,7 +579,7 @@ static StageCoeffs ssp3_coeffs(int stage) {
 static int frozen(const zfv_handle *h) { return h->dimg.steady != nullptr ? 1 : 0; }
 
 // Autotuner of the reconstruction launch configuration: every candidate (1,
-// 2, 4 waves per block, with and without the occupancy hint) is launched on
+// 2, 4 waves per block) is launched on
 // the instance's own state, one warm-up and five timed launches between HIP
 // events on the handle's stream; the fastest average wins.  The launches only
 // write the face-point and source buffers.
@@ -611,11 +608,6 @@ static void tune_rec(zfv_handle *h, const double *Ustate) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   h->rec_cfg = best;
-}
-
-// the narrowest physics set whose kernel covers this instance
-static int physics_set(const Img &m) {
-  return (m.eq == 0 || m.eq == 2) && (m.gravity == 0 || m.gravity == 1) ? PH_LIGHT : PH_FULL;
 }
 
 // reconstruction launch configuration, tuning on first use (never inside a capture)
@@ -660,12 +652,11 @@ static void from_soa(const zfv_handle *h, const double *soa, double *aos, bool a
   }
 }
 
-// launch configuration named by (threads per block, occupancy hint)
-static int rec_cfg_of(int32_t rec_threads, int32_t occupancy_hint) {
+// launch configuration named by its threads per block
+static int rec_cfg_of(int32_t rec_threads) {
   if (rec_threads != 64 && rec_threads != 128 && rec_threads != 256)
     throw ZfvError(ZFV_ERR_INVALID, "rec_threads must be 0, -1, 64, 128 or 256");
-  if (occupancy_hint != 0 && occupancy_hint != 4) throw ZfvError(ZFV_ERR_INVALID, "occupancy_hint must be 0 or 4");
-  return (occupancy_hint ? 3 : 0) + (rec_threads == 64 ? 0 : (rec_threads == 128 ? 1 : 2));
+  return rec_threads == 64 ? 0 : (rec_threads == 128 ? 1 : 2);
 }
 
 static void report_tuning(int cfg, int groups_per_wave, const double *ms, zfv_tuning *out) {
@@ -673,7 +664,6 @@ static void report_tuning(int cfg, int groups_per_wave, const double *ms, zfv_tu
   const int c = cfg < 0 ? kRecConfigs - 1 : cfg;
   out->rec_threads = 64 * rec_cfg_waves(c);
   out->cells_per_block = rec_cfg_waves(c) * groups_per_wave;
-  out->occupancy_hint = rec_cfg_occ(c) ? 4 : 0;
   out->tuned = cfg >= 0;
   for (int k = 0; k < kRecConfigs; ++k) out->rec_ms[k] = ms[k];
 }
@@ -731,7 +721,6 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.flux_bc = params->flux_bc;
   for (int q = 0; q < 16; ++q)
     for (int d = 0; d < 3; ++d) m.shift[q][d] = h->img.shift[3 * q + d];
-  h->ph = physics_set(m);
   if (im.face_w.size() > 8 || im.vol_w.size() > 16) throw ZfvError(ZFV_ERR_INVALID, "quadrature rule too large");
   const int fdim = s.dim == 2 ? 1 : 3;
   for (size_t q = 0; q < im.face_w.size(); ++q) {
@@ -763,6 +752,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     HIP_CHECK(hipStreamSynchronize(h->stream));
     m.phim = pm;
   }
+  h->ph = physics_set(m);
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
   *out = h.release();
   return ZFV_OK;
@@ -915,16 +905,19 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
     maxs = std::max(maxs, h->img.l2g_off[i + 1] - h->img.l2g_off[i]);
   info->max_stencil_size = maxs;
   info->n_points_face = s.q;
+  int64_t far = 0;
+  for (int32_t v : h->img.meta) far += (v & zfv::META_FAR_GHOST) == zfv::META_FAR_GHOST;
+  info->n_far_ghost_cells = far;
   return ZFV_OK;
   ZFV_API_END
 }
 
-int zfv_tune(zfv_handle *h, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out) {
+int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out) {
   ZFV_API_BEGIN
   if (rec_threads == 0) {
     tune_rec(h, h->U);
   } else if (rec_threads > 0) {
-    h->rec_cfg = rec_cfg_of(rec_threads, occupancy_hint);
+    h->rec_cfg = rec_cfg_of(rec_threads);
   }
   if (h->graph_exec && rec_threads >= 0) {  // the captured step holds the old configuration
     HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
@@ -1085,7 +1078,7 @@ struct zfv_batch {
   int64_t n_cell_blocks = 0, n_face_blocks = 0, n_rec_blocks = 0;  // current launch sizes
   bool xcd_chunks = true;
   int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
-  double rec_ms[6] = {0, 0, 0, 0, 0, 0};
+  double rec_ms[zfv::kRecConfigs] = {0, 0, 0};
   int ph = 0;         // physics set: the widest any member needs
   zfv::DevInst *d_inst = nullptr;
   zfv::BlockRef *d_bcell = nullptr, *d_bface = nullptr, *d_brec = nullptr;
@@ -1148,9 +1141,13 @@ static void batch_upload(zfv_batch *b) {
       cells[s] = b->h_inst[s].m.n;
       faces[s] = b->h_inst[s].m.ne;
     }
-  int ph = PH_LIGHT;
+  int ph = -1;
   for (int s = 0; s < b->max_inst; ++s)
-    if (b->members[s] && b->h_inst[s].active) ph = std::max(ph, physics_set(b->h_inst[s].m));
+    if (b->members[s] && b->h_inst[s].active) {
+      const int p = physics_set(b->h_inst[s].m);
+      ph = ph < 0 ? p : physics_union(ph, p);
+    }
+  if (ph < 0) ph = PH_LIGHT;  // no active member: nothing is launched
   if (ph != b->ph && b->graph) {
     HIP_CHECK(hipGraphExecDestroy(b->graph));
     b->graph = nullptr;
@@ -1353,13 +1350,13 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   ZFV_API_END
 }
 
-int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, int32_t occupancy_hint, zfv_tuning *out) {
+int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out) {
   ZFV_API_BEGIN
   if (b->dirty) batch_upload(b);
   if (rec_threads == 0) {
     if (batch_has_active(b)) batch_tune(b);
   } else if (rec_threads > 0) {
-    b->rec_cfg = rec_cfg_of(rec_threads, occupancy_hint);
+    b->rec_cfg = rec_cfg_of(rec_threads);
     batch_upload(b);
   }
   if (b->graph && rec_threads >= 0) {

@@ -89,11 +89,16 @@ ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
 // ---- gravity -----------------------------------------------------------------
 // PH selects the physics a kernel is compiled for: PH_LIGHT covers no or axial
 // gravity with no or isothermal equilibrium (exp only); PH_FULL adds radial
-// and polytrope gravity and the isentropic equilibrium.  The transcendental
-// functions of the full set need many registers, so runs that do not need
-// them use kernels without them.
+// and polytrope gravity and the isentropic equilibrium.  PH_LIN is PH_FULL
+// restricted to the isentropic equilibrium of gamma = 2 with potential
+// moments and no periodic images, where a member's equilibrium average needs
+// only the member's two potential moments (linear_isentropic_avg), not its
+// corners.  The transcendental functions of the full set and the member
+// corners need many registers, so runs that do not need them use kernels
+// without them.
 constexpr int PH_LIGHT = 0;
 constexpr int PH_FULL = 1;
+constexpr int PH_LIN = 2;
 
 template <int DIM, int PH = PH_FULL>
 ZFV_HD double phi(const Img &m, const double *x) {
@@ -140,7 +145,7 @@ ZFV_HD void grad_phi(const Img &m, const double *x, double *gp) {
 template <int DIM, int PH = PH_FULL>
 ZFV_HD void eq_eval(const Img &m, const double *th, double phi_ref, const double *x, double &rho, double &E) {
   const double gamma = m.gamma;
-  if (PH == PH_FULL && m.eq == 1) {
+  if (PH != PH_LIGHT && m.eq == 1) {
     double h = th[0] + phi_ref - phi<DIM, PH>(m, x);
     double K = th[1];
     double base = 1.0 / K * (gamma - 1.0) / gamma * h;
@@ -169,7 +174,7 @@ ZFV_HD EqC eq_consts(const Img &m, const double *th, double phi_ref) {
   c.th0 = th[0];
   c.th1 = th[1];
   c.phi_ref = phi_ref;
-  if (PH == PH_FULL && m.eq == 1) {
+  if (PH != PH_LIGHT && m.eq == 1) {
     c.a = 1.0 / th[1] * (m.gamma - 1.0) / m.gamma;
     c.b = 1.0 / (m.gamma - 1.0);
   } else {
@@ -191,7 +196,7 @@ ZFV_HD double ipow(double x, double y) {
 template <int DIM, int PH>
 ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, double &E) {
   const double ph = phi<DIM, PH>(m, x);
-  if (PH == PH_FULL && m.eq == 1) {
+  if (PH != PH_LIGHT && m.eq == 1) {
     double h = c.th0 + c.phi_ref - ph;
     rho = ipow(c.a * h, c.b);
     double p = c.th1 * ipow(rho, m.gamma);
@@ -209,6 +214,21 @@ ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, d
 // with a = 1/(2K), A = th0 + phi_ref.  Exact in real arithmetic for the same
 // quadrature rule; moments are per cell and time independent.
 ZFV_HD bool linear_isentropic(const Img &m) { return m.eq == 1 && m.gamma == 2.0 && m.phim != nullptr; }
+
+// the narrowest physics set whose kernel covers this instance
+inline int physics_set(const Img &m) {
+  if ((m.eq == 0 || m.eq == 2) && (m.gravity == 0 || m.gravity == 1)) return PH_LIGHT;
+  if (linear_isentropic(m)) {
+    bool shifted = false;  // a periodic image among the stencil members
+    for (int c = 1; c < 16; ++c)
+      for (int d = 0; d < 3; ++d) shifted = shifted || m.shift[c][d] != 0.0;
+    if (!shifted) return PH_LIN;
+  }
+  return PH_FULL;
+}
+
+// the physics set of a kernel that runs instances of sets a and b
+inline int physics_union(int a, int b) { return a == b ? a : PH_FULL; }
 
 ZFV_HD void linear_isentropic_avg(const EqC &c, double phim, double phiv, double &ar, double &aE) {
   const double h = c.th0 + c.phi_ref - phim;
@@ -496,7 +516,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // then every lane adds the NV partial sums in lane order, so all lanes hold
   // the same value.
   auto group_avg = [&](const EqC &tc, double &ar, double &aE) {
-    if (PH == PH_FULL && linear_isentropic(m)) {
+    if (PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m))) {
       linear_isentropic_avg(tc, m.phim[i], m.phim[n + i], ar, aE);
       return;
     }
@@ -618,22 +638,21 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double pv[DIM + 1];
       for (int k = 0; k < DIM + 1; ++k) pv[k] = phi<DIM, PH>(m, w3[k]);
       // two points per iteration: independent exponentials interleave; the
-      // sum still runs in point order
+      // sum still runs in point order.  An odd last point pairs with itself
+      // at weight zero (adding +0 leaves the sum unchanged), so the loop body
+      // is the only copy of the exponential.
       auto point = [&](int q) {
         double ph = m.vx[q][0] * pv[0];
         for (int k = 1; k < DIM + 1; ++k) ph = ph + m.vx[q][k] * pv[k];
         return exp((ph - thc.phi_ref) * thc.a);
       };
       double se = 0.0;
-      int q = 0;
-      for (; q + 1 < m.qv; q += 2) {
-        const double e0 = point(q), e1 = point(q + 1);
+      for (int q = 0; q < m.qv; q += 2) {
+        const bool pair = q + 1 < m.qv;
+        const double e0 = point(q), e1 = point(pair ? q + 1 : q);
+        const double w1 = pair ? m.vw[q + 1] : 0.0;
         se = (q == 0) ? m.vw[q] * e0 : se + m.vw[q] * e0;
-        se = se + m.vw[q + 1] * e1;
-      }
-      if (q < m.qv) {
-        const double e0 = point(q);
-        se = (q == 0) ? m.vw[q] * e0 : se + m.vw[q] * e0;
+        se = se + w1 * e1;
       }
       er = thc.th0 * se;
       eE = er * thc.b;
@@ -694,13 +713,20 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // are unrolled so that their coefficients stay in registers.
     double *pl = xch.lds();  // NV rows x NC coefficients
     // scaled deviation of row k of the chunk in this lane's variable
+    // (the equilibrium parts enter with factors 1 or 0: u - 1 er = u - er
+    // and x - 0 eE = x exactly, so this equals the branchy form bit for bit)
+    const double f_rho = (wb && is_rho) ? 1.0 : 0.0, f_E = (wb && is_E) ? 1.0 : 0.0;
     auto deviation = [&](double u, double er, double eE) {
-      double d = u;
-      if (wb) {
-        if (is_rho) d -= er;
-        if (is_E) d -= eE;
-      }
+      const double d = (u - f_rho * er) - f_E * eE;
       return d * inv_g - q0;
+    };
+    // a member's equilibrium averages as lane k computed them; in the light
+    // set the energy average is the density average times the cell constant
+    // b (member_avg), formed here from the broadcast density average
+    auto member_eq = [&](double my_er, double my_eE, int k, double &er, double &eE) {
+      er = xch.bcast(my_er, k);
+      if (PH == PH_LIGHT) eE = eq_sub ? er * thc.b : 0.0;
+      else eE = xch.bcast(my_eE, k);
     };
 
     // translated corners and volume of a member (row entry rw)
@@ -716,8 +742,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     auto load_member = [&](int32_t rw, Member &mb) {
       const int64_t j = rw & ROW_CELL_MASK;
       const int code = rw >> ROW_CELL_BITS;
-      mb.lin = (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
-      if (mb.lin) {
+      mb.lin = PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
+      if (PH == PH_LIN || mb.lin) {
         mb.pm = m.phim[j];
         mb.pv = m.phim[n + j];
         return;
@@ -728,7 +754,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       mb.vol = m.vol[j];
     };
     auto member_avg_of = [&](const Member &mb, double &er, double &eE) {
-      if (PH == PH_FULL && mb.lin) linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
+      if (PH == PH_LIN || (PH == PH_FULL && mb.lin)) linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
       else member_avg(mb.w3, mb.vol, er, eE);
     };
     // central rows: the row entries run two chunks ahead and lane g's member
@@ -760,8 +786,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       for (int k = 0; k < NV; ++k) {
         double er = 0.0, eE = 0.0;
         if (wb) {
-          er = xch.bcast(my_er, k);
-          eE = xch.bcast(my_eE, k);
+          member_eq(my_er, my_eE, k, er, eE);
         }
         if (r0 + k < RC) {
           const double d = deviation(uk[k], er, eE);
@@ -808,8 +833,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         for (int k = 0; k < NV; ++k) {
           double er = 0.0, eE = 0.0;
           if (wb) {
-            er = xch.bcast(my_er, k);
-            eE = xch.bcast(my_eE, k);
+            member_eq(my_er, my_eE, k, er, eE);
           }
           if (c0 + k < RB) {
             const double d = deviation(uk[k], er, eE);
