@@ -28,7 +28,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim;
+  std::vector<double> U, fpv, G, src, phim, pot;
 };
 
 // The lanes of one cell's group run as NV host threads in lockstep: every
@@ -165,6 +165,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.fperm = im.fperm.data();
     m.steady = nullptr;
     m.phim = nullptr;
+    m.pot = nullptr;
     m.gamma = p->gamma;
     m.eps = p->epsilon;
     m.exponent = p->exponent;
@@ -200,6 +201,18 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
           phi_moments<3>(m, i, e->phim[i], e->phim[im.n + i]);
       }
       m.phim = e->phim.data();
+    }
+    if (physics_set(m) != PH_LIGHT && p->gravity != ZFV_GRAVITY_NONE) {
+      // the potential tables the GPU path keeps for these physics sets
+      const int rows = im.s.dim == 2 ? pot_rows<2>(m.qv, im.s.q) : pot_rows<3>(m.qv, im.s.q);
+      e->pot.assign(size_t(rows) * im.n, 0.0);
+      for (int64_t i = 0; i < im.n; ++i) {
+        if (im.s.dim == 2)
+          potential_points<2>(m, im.s.q, i, e->pot.data());
+        else
+          potential_points<3>(m, im.s.q, i, e->pot.data());
+      }
+      m.pot = e->pot.data();
     }
     const int nv = im.s.nvars;
     e->fpv.assign(size_t(2) * im.s.q * nv * std::max<int64_t>(im.ne, 1), 0.0);

@@ -179,6 +179,14 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// per-cell potential tables of the cells' own quadrature points
+template <int DIM>
+__global__ __launch_bounds__(256) void k_potential_points(Img m, int nqf, double *tab) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  potential_points<DIM>(m, nqf, i, tab);
+}
+
 // per-cell potential moments for the linear isentropic equilibrium averages
 template <int DIM>
 __global__ __launch_bounds__(256) void k_phi_moments(Img m, double *out) {
@@ -705,6 +713,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.fperm = h->upload(im.fperm);
   m.steady = nullptr;
   m.phim = nullptr;
+  m.pot = nullptr;
   m.gamma = params->gamma;
   m.eps = params->epsilon;
   m.exponent = params->exponent;
@@ -753,6 +762,17 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     m.phim = pm;
   }
   h->ph = physics_set(m);
+  if (h->ph != PH_LIGHT && params->gravity != ZFV_GRAVITY_NONE) {
+    // potential tables of the cells' own quadrature points
+    double *tab = h->alloc<double>(size_t(s.dim == 2 ? pot_rows<2>(m.qv, s.q) : pot_rows<3>(m.qv, s.q)) * n);
+    if (s.dim == 2)
+      hipLaunchKernelGGL(k_potential_points<2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, s.q, tab);
+    else
+      hipLaunchKernelGGL(k_potential_points<3>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, s.q, tab);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    m.pot = tab;
+  }
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
   *out = h.release();
   return ZFV_OK;

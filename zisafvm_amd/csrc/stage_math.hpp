@@ -57,6 +57,7 @@ struct Img {
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
+  const double *pot;     // potential tables of the cell's own points (pot_index), or null
   // physics
   double gamma;
   double eps, exponent;
@@ -193,18 +194,24 @@ ZFV_HD double ipow(double x, double y) {
   return pow(x, y);
 }
 
-template <int DIM, int PH>
-ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, double &E) {
-  const double ph = phi<DIM, PH>(m, x);
+// the equilibrium at a point of potential ph
+template <int PH>
+ZFV_HD void eq_at_phi(const Img &m, const EqC &c, double ph, double &rho, double &E) {
   if (PH != PH_LIGHT && m.eq == 1) {
     double h = c.th0 + c.phi_ref - ph;
-    rho = ipow(c.a * h, c.b);
-    double p = c.th1 * ipow(rho, m.gamma);
+    // PH_LIN has gamma = 2: the exponents are 1 and 2, as ipow takes them
+    rho = PH == PH_LIN ? c.a * h : ipow(c.a * h, c.b);
+    double p = c.th1 * (PH == PH_LIN ? rho * rho : ipow(rho, m.gamma));
     E = p / (m.gamma - 1.0);
   } else {
     rho = c.th0 * exp((ph - c.phi_ref) * c.a);
     E = rho * c.b;
   }
+}
+
+template <int DIM, int PH>
+ZFV_HD void eq_point(const Img &m, const EqC &c, const double *x, double &rho, double &E) {
+  eq_at_phi<PH>(m, c, phi<DIM, PH>(m, x), rho, E);
 }
 
 // The isentropic equilibrium of an ideal gas with gamma = 2 is linear in the
@@ -253,6 +260,17 @@ ZFV_HD void face_vertices(uint32_t fperm, int f, const double (*v)[3], double (*
     w[r][0] = v[c][0];
     w[r][1] = v[c][1];
     w[r][2] = v[c][2];
+  }
+}
+
+// the same vertices read from the cell's corner arrays (cache hits) instead
+// of selected from registers
+template <int DIM>
+ZFV_HD void load_face_vertices(const Img &m, int64_t i, uint32_t fperm, int f, double (*w)[3]) {
+  for (int r = 0; r < DIM; ++r) {
+    const int c = (fperm >> (2 * (f * 3 + r))) & 3u;
+    for (int d = 0; d < DIM; ++d) w[r][d] = m.crn[(c * DIM + d) * m.n + i];
+    if (DIM == 2) w[r][2] = 0.0;
   }
 }
 
@@ -342,6 +360,47 @@ ZFV_HD void phi_moments(const Img &m, int64_t i, double &mean, double &var) {
   }
   mean = s;
   var = v;
+}
+
+// Potential tables.  Radial and polytrope gravity have no affine closed form,
+// and the potential and its gradient at a cell's own quadrature points do not
+// change in time, so instances of the full physics sets with gravity keep
+// them per cell (SoA, n cells), computed once with the functions below:
+//   pot[q n + i]                       phi at volume point q      (q < qv)
+//   pot[(qv + q DIM + d) n + i]        d-th component of grad phi there
+//   pot[(qv (1 + DIM) + f Q + q) n + i] phi at point q of local face f
+// The reconstruction then evaluates no transcendental function at its own
+// points; the values are those the functions give at the same points.
+template <int DIM>
+ZFV_HD int64_t pot_index_grad(const Img &m, int q, int d) { return int64_t(m.qv + q * DIM + d) * m.n; }
+template <int DIM>
+ZFV_HD int64_t pot_index_face(const Img &m, int nqf, int f, int q) { return int64_t(m.qv * (1 + DIM) + f * nqf + q) * m.n; }
+// rows of the tables for nqf face points
+template <int DIM>
+ZFV_HD int pot_rows(int qv, int nqf) { return qv * (1 + DIM) + (DIM + 1) * nqf; }
+
+// fills the potential tables of cell i (tab has pot_rows * n entries)
+template <int DIM>
+ZFV_HD void potential_points(const Img &m, int nqf, int64_t i, double *tab) {
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+  for (int q = 0; q < m.qv; ++q) {
+    double x[3], gp[3];
+    volume_point<DIM>(m, q, vtx, x);
+    tab[int64_t(q) * m.n + i] = phi<DIM, PH_FULL>(m, x);
+    grad_phi<DIM, PH_FULL>(m, x, gp);
+    for (int d = 0; d < DIM; ++d) tab[pot_index_grad<DIM>(m, q, d) + i] = gp[d];
+  }
+  const uint32_t fperm = m.fperm[i];
+  for (int f = 0; f < DIM + 1; ++f) {
+    double w3[3][3];
+    face_vertices<DIM>(fperm, f, vtx, w3);
+    for (int q = 0; q < nqf; ++q) {
+      double x[3];
+      face_point<DIM>(m, q, w3, x);
+      tab[pot_index_face<DIM>(m, nqf, f, q) + i] = phi<DIM, PH_FULL>(m, x);
+    }
+  }
 }
 
 // ---- polynomial evaluation (include/zisa/math/poly2d_impl.hpp:271-323) -------
@@ -511,6 +570,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   double vtx[DIM + 1][3];
   load_corners<DIM>(m, i, vtx);
 
+  // The potential at the cell's own quadrature points: from the instance's
+  // tables in the full sets (always present in PH_LIN), else evaluated.
+  const bool tab = PH == PH_LIN || (PH == PH_FULL && m.pot != nullptr);
+  auto vol_phi = [&](int q, const double *x) { return tab ? m.pot[int64_t(q) * n + i] : phi<DIM, PH>(m, x); };
+
   // Average of an equilibrium over this cell with the quadrature points split
   // over the group's lanes: lane k sums the points q = k (mod NV) in order,
   // then every lane adds the NV partial sums in lane order, so all lanes hold
@@ -526,7 +590,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double x[3];
       volume_point<DIM>(m, q, vtx, x);
       double r, E;
-      eq_point<DIM, PH>(m, tc, x, r, E);
+      eq_at_phi<PH>(m, tc, vol_phi(q, x), r, E);
       const double w = vol_i * m.vw[q];
       pr = pr + w * r;
       pE = pE + w * E;
@@ -548,7 +612,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   if (wb) {
     double xref[3];
     volume_point<DIM>(m, 0, vtx, xref);
-    phi_ref = phi<DIM, PH>(m, xref);
+    phi_ref = vol_phi(0, xref);
     double vol_i = m.vol[i];
     // cell average of the equilibrium with parameters t over this cell
     auto cell_avg = [&](const double *t, double &ar, double &aE) {
@@ -566,7 +630,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       // quasi-Newton with a central finite-difference Jacobian
       // (include/zisa/math/quasi_newton.hpp:12-50)
       double p = Eint0 * (gamma - 1.0);
-      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / ipow(rho0, gamma)};
+      double guess[2] = {gamma / (gamma - 1.0) * p / rho0, p / (PH == PH_LIN ? rho0 * rho0 : ipow(rho0, gamma))};
       double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
       double x0 = guess[0], x1 = guess[1];
       auto f = [&](double a, double b, double &f0, double &f1) {
@@ -697,10 +761,10 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // this variable's coefficients: central c[alpha] (alpha = 1..DOFC-1),
   // one-sided b[k][alpha] (alpha = 1..DIM)
   double c[NC];
+  // (b is zeroed after the central rows, where it starts to be used, so it
+  // holds no registers during them)
   double b[NB > 0 ? NB : 1][NBC];
   for (int a = 0; a < NC; ++a) c[a] = 0.0;
-  for (int k = 0; k < NB; ++k)
-    for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
 
   if (!first_order) {
     // Rows in chunks of NV: lane k of the group owns row r0 + k of a chunk.
@@ -782,23 +846,26 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 #pragma unroll
       for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
       xch.sync();
-#pragma unroll
+      // Every row of the chunk is accumulated, also those past the last
+      // (r0 + k >= RC): their LDS entries are 0 and their deviation is finite
+      // (member cell 0, equilibrium averages 0), and fma(0, d, c) = c exactly
+      // for an accumulator that started at +0.  A rolled loop over k keeps a
+      // single row's coefficients in registers at a time.
+#pragma unroll 1
       for (int k = 0; k < NV; ++k) {
         double er = 0.0, eE = 0.0;
-        if (wb) {
-          member_eq(my_er, my_eE, k, er, eE);
-        }
-        if (r0 + k < RC) {
-          const double d = deviation(uk[k], er, eE);
+        if (wb) member_eq(my_er, my_eE, k, er, eE);
+        const double d = deviation(uk[k], er, eE);
 #pragma unroll
-          for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
-        }
+        for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
       }
       rw_cur = rw_nxt;
       rw_nxt = rw_far;
       mb_cur = mb_nxt;
     }
 
+    for (int k = 0; k < NB; ++k)
+      for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
     // one-sided stencils: rows RC + kb * RB + rr (a rolled loop; the target
     // stencil's coefficients are picked with compile-time indices)
     rw_cur = g < RB ? m.rows[(RC + g) * n + i] : int32_t(0);
@@ -835,15 +902,15 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
           if (wb) {
             member_eq(my_er, my_eE, k, er, eE);
           }
-          if (c0 + k < RB) {
-            const double d = deviation(uk[k], er, eE);
+          // rows past the stencil's last (c0 + k >= RB) add exact zeros, as
+          // in the central loop
+          const double d = deviation(uk[k], er, eE);
 #pragma unroll
-            for (int a = 0; a < NBC; ++a) {
-              const double p = pl[k * NC + a];
+          for (int a = 0; a < NBC; ++a) {
+            const double p = pl[k * NC + a];
 #pragma unroll
-              for (int kk = 0; kk < NB; ++kk)
-                if (kk == kb) b[kk][a] = fma(p, d, b[kk][a]);
-            }
+            for (int kk = 0; kk < NB; ++kk)
+              if (kk == kb) b[kk][a] = fma(p, d, b[kk][a]);
           }
         }
         rw_cur = rw_next;
@@ -949,7 +1016,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   for (int f = 0; f < NF; ++f) {
     const int32_t cf = m.cface[f * n + i];
     double w3[3][3];
-    face_vertices<DIM>(fperm, f, vtx, w3);
+    load_face_vertices<DIM>(m, i, fperm, f, w3);
     double nn[3], t1[3], t2[3], area;
     face_frame<DIM>(w3, nn, t1, t2, area);
     const bool exterior = (cf & 2) != 0;
@@ -973,7 +1040,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         if (q0 + g < Q) {
           double x[3];
           face_point<DIM>(m, q0 + g, w3, x);
-          eq_point<DIM, PH>(m, thc, x, my_r, my_E);
+          const double ph = tab ? m.pot[pot_index_face<DIM>(m, Q, f, q0 + g) + i] : phi<DIM, PH>(m, x);
+          eq_at_phi<PH>(m, thc, ph, my_r, my_E);
         }
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
@@ -1015,6 +1083,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   }
 
   if (gravity_src) {
+    // the cell's corners again (from cache): not held in registers through
+    // the row and face loops above
+    load_corners<DIM>(m, i, vtx);
     const double vol_i = m.vol[i];
     double cs_ = 0.0;
     for (int q0 = 0; q0 < m.qv; q0 += NV) {
@@ -1023,8 +1094,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (eq_sub && q0 + g < m.qv) {
         double x[3];
         volume_point<DIM>(m, q0 + g, vtx, x);
-        eq_point<DIM, PH>(m, thc, x, my_r, my_E);
+        eq_at_phi<PH>(m, thc, vol_phi(q0 + g, x), my_r, my_E);
       }
+#pragma unroll 1
       for (int k = 0; k < NV; ++k) {
         const int q = q0 + k;
         double r_eq = 0.0, E_eq = 0.0;
@@ -1038,7 +1110,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         double u, dl;
         eval(x, r_eq, E_eq, u, dl);
         double gp[3];
-        grad_phi<DIM, PH>(m, x, gp);
+        if (tab) {
+          for (int d = 0; d < DIM; ++d) gp[d] = m.pot[pot_index_grad<DIM>(m, q, d) + i];
+        } else {
+          grad_phi<DIM, PH>(m, x, gp);
+        }
         const double drho = xch.bcast(wb ? dl : u, 0);
         double mc[3];
         for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
