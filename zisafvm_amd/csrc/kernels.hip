@@ -77,7 +77,9 @@ __global__ __launch_bounds__(256) void k_reconstruct(
     Img m, const double *__restrict__ U, double *__restrict__ fpv, double *__restrict__ src, int skip_far_ghosts) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
-  __shared__ double slices[4][GPW][NV * (DOFC - 1)];
+  // one slice of NV * (DOFC - 1) doubles per cell group, sized at launch
+  // (rec_lds_bytes) for the block's waves
+  extern __shared__ double slices[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int grp = lane / NV;
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(
   // under the frozen BC a ghost cell away from the interior feeds no flux and
   // its own tendency is overwritten: nothing to reconstruct
   if (skip_far_ghosts && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  WaveGroup x{lane - g, &slices[wave][grp][0]};
+  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * (NV * (DOFC - 1))]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
 }
 
@@ -225,7 +227,7 @@ template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
-  __shared__ double slices[4][GPW][NV * (DOFC - 1)];
+  extern __shared__ double slices[];  // as in k_reconstruct
   const BlockRef b = blk[blockIdx.x];
   if (b.inst < 0) return;
   const DevInst &d = insts[b.inst];
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const Bloc
   const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= d.m.n) return;
   if (d.m.steady != nullptr && (d.m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  WaveGroup x{lane - g, &slices[wave][grp][0]};
+  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * (NV * (DOFC - 1))]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
 }
 
@@ -404,19 +406,26 @@ inline unsigned xcd_grid(int64_t n, int bs) { return (blocks_for(n, bs) + 7u) / 
 // spilled to scratch and were never faster; they are not built.)
 constexpr int kRecConfigs = 3;
 inline int rec_cfg_waves(int cfg) { return 1 << cfg; }
+// LDS of a reconstruction block of `waves` wavefronts: one slice per cell group
+template <int DIM, int DOFC>
+inline unsigned rec_lds_bytes(int waves) {
+  return unsigned(waves * groups_per_wave<DIM + 2>() * (DIM + 2) * (DOFC - 1) * sizeof(double));
+}
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
   const int cpb = waves * groups_per_wave<DIM + 2>();
-  hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), 0, s,
+  const unsigned lds = rec_lds_bytes<DIM, DOFC>(waves);
+  hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), lds, s,
                      m, U, fpv, src, skip);
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_brec(DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
-  hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(nb), dim3(64 * waves), 0, s, d, b, stage);
+  const unsigned lds = rec_lds_bytes<DIM, DOFC>(waves);
+  hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(nb), dim3(64 * waves), lds, s, d, b, stage);
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
@@ -911,11 +920,16 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   info->device_bytes = h->device_bytes;
   // algorithmic bytes of the reconstruction kernel per cell: pseudo-inverse,
   // stencil indices, own state, geometry (corners, centre, length, volume,
-  // moments), meta/face words, and the face-point values written once per
-  // interior face side
+  // moments), meta/face words, the potential tables and moments where the
+  // instance has them, the face-point values written once per interior face
+  // side and the source term written (with gravity)
   const double nv = s.nvars;
+  const Img &dm = h->dimg;
+  const double tables = (dm.pot ? 8.0 * (s.dim == 2 ? pot_rows<2>(dm.qv, s.q) : pot_rows<3>(dm.qv, s.q)) : 0.0) +
+                        (dm.phim ? 16.0 : 0.0);
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
-                     4.0 * (1 + s.nfaces + 1) + 8.0 * (2.0 * ne / n) * s.q * nv;
+                     4.0 * (1 + s.nfaces + 1) + tables + 8.0 * (2.0 * ne / n) * s.q * nv +
+                     (h->src ? 8.0 * nv : 0.0);
   const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv);
   const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * s.dim + 1) + 4.0 + 8.0 * nv * 5;
   info->recon_bytes_per_cell = rec;
