@@ -60,6 +60,28 @@ def test_equilibrium_stays_at_rest(gpu, name, make, n_steps):
     assert max(drift) <= 1e-12
 
 
+@pytest.mark.parametrize(
+    "name,make,n_steps",
+    [
+        # the full-size well-balanced configurations: the isothermal atmosphere
+        # on the 360 x 360 lattice (263,520 cells) and the polytrope on the
+        # ~2M-tetrahedron ball, over the reference's 50 steps
+        ("isothermal2d_n360", lambda: W.isothermal_2d(n=360, amplitude=0.0), 50),
+        ("polytrope3d_n86", lambda: W.polytrope_3d(n=86, amplitude=0.0), 50),
+    ],
+)
+def test_equilibrium_stays_at_rest_at_full_size(gpu, name, make, n_steps):
+    wl = make()
+    print(name, wl.mesh.n_cells, "cells", flush=True)
+    s = z.Solver(wl.mesh, wl.params)
+    s.set_state(wl.u0)
+    s.set_steady_state(wl.steady)
+    s.steps(n_steps, 0.0, s.cfl_dt(0.45))
+    drift = _drift(wl, s.get_state())
+    print(name, n_steps, "steps: drift (rho, momentum, E)", drift)
+    assert max(drift) <= 1e-12
+
+
 def _vortex(n, order):
     mesh = z.lattice_mesh_2d(n, jitter=0.2, seed=1, periodic=(True, True))
     p = z.make_params(n_dims=2, order=order, flux=z.ZFV_FLUX_HLLC)

@@ -821,36 +821,59 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (PH == PH_LIN || (PH == PH_FULL && mb.lin)) linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
       else member_avg(mb.w3, mb.vol, er, eE);
     };
-    // central rows: the row entries run two chunks ahead and lane g's member
-    // one chunk ahead of the chunk being accumulated
-    int32_t rw_cur = g < RC ? m.rows[g * n + i] : int32_t(0);
-    int32_t rw_nxt = NV + g < RC ? m.rows[(NV + g) * n + i] : int32_t(0);
+    // All rows R = RC + NB RB (central rows, then the one-sided stencils'
+    // rows kb RB + rr) in one stream of chunks: the row entries run two chunks
+    // ahead and lane g's member one chunk ahead of the chunk being
+    // accumulated.  Chunks of central rows only run in a first rolled loop,
+    // the rest (the last central rows and the one-sided ones) in a second, so
+    // the one-sided coefficients b hold registers only there.
+    constexpr int R = RC + NB * RB;
+    int32_t rw_cur = g < R ? m.rows[g * n + i] : int32_t(0);
+    int32_t rw_nxt = NV + g < R ? m.rows[(NV + g) * n + i] : int32_t(0);
     Member mb_cur{};
-    if (eq_sub && g < RC) load_member(rw_cur, mb_cur);
-#pragma unroll 1
-    for (int r0 = 0; r0 < RC; r0 += NV) {
+    if (eq_sub && g < R) load_member(rw_cur, mb_cur);
+    // stages chunk r0: loads the chunk's member states, advances the
+    // member/row pipeline, computes this lane's member average, then loads
+    // this lane's row of the pseudo-inverse (central: NC entries; one-sided:
+    // NBC, the rest 0; loaded after the average so that they hold no
+    // registers during it) and shares the rows through the LDS slice
+    auto chunk = [&](int r0, double *uk, double &my_er, double &my_eE) {
       const int my_r = r0 + g;
-      const bool mine = my_r < RC;
-      double uk[NV];
+      const bool mine = my_r < R;
 #pragma unroll
       for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
-      double pr[NC];
-#pragma unroll
-      for (int a = 0; a < NC; ++a) pr[a] = mine ? m.P[(a * RC + my_r) * n + i] : 0.0;
       Member mb_nxt{};
-      if (eq_sub && my_r + NV < RC) load_member(rw_nxt, mb_nxt);
-      const int32_t rw_far = my_r + 2 * NV < RC ? m.rows[(my_r + 2 * NV) * n + i] : int32_t(0);
-      double my_er = 0.0, my_eE = 0.0;
+      if (eq_sub && my_r + NV < R) load_member(rw_nxt, mb_nxt);
+      const int32_t rw_far = my_r + 2 * NV < R ? m.rows[(my_r + 2 * NV) * n + i] : int32_t(0);
+      my_er = 0.0;
+      my_eE = 0.0;
       if (eq_sub && mine) member_avg_of(mb_cur, my_er, my_eE);
+      double pr[NC];
+      if (my_r < RC) {
+#pragma unroll
+        for (int a = 0; a < NC; ++a) pr[a] = m.P[(a * RC + my_r) * n + i];
+      } else {
+        const int kb = (my_r - RC) / RB, rr = (my_r - RC) - kb * RB;
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+          pr[a] = (mine && a < NBC) ? m.P[(NC * RC + kb * NBC * RB + a * RB + rr) * n + i] : 0.0;
+      }
       xch.sync();
 #pragma unroll
       for (int a = 0; a < NC; ++a) pl[g * NC + a] = pr[a];
       xch.sync();
-      // Every row of the chunk is accumulated, also those past the last
-      // (r0 + k >= RC): their LDS entries are 0 and their deviation is finite
-      // (member cell 0, equilibrium averages 0), and fma(0, d, c) = c exactly
-      // for an accumulator that started at +0.  A rolled loop over k keeps a
-      // single row's coefficients in registers at a time.
+      rw_cur = rw_nxt;
+      rw_nxt = rw_far;
+      mb_cur = mb_nxt;
+    };
+    // Rows are accumulated in a rolled loop over the chunk (one row's
+    // entries in registers at a time).  Central rows past the last of a
+    // chunk would add exact zeros; rows past R are skipped.
+    constexpr int R0 = (RC / NV) * NV;  // central rows in whole chunks
+#pragma unroll 1
+    for (int r0 = 0; r0 < R0; r0 += NV) {
+      double uk[NV], my_er, my_eE;
+      chunk(r0, uk, my_er, my_eE);
 #pragma unroll 1
       for (int k = 0; k < NV; ++k) {
         double er = 0.0, eE = 0.0;
@@ -859,52 +882,25 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 #pragma unroll
         for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
       }
-      rw_cur = rw_nxt;
-      rw_nxt = rw_far;
-      mb_cur = mb_nxt;
     }
-
     for (int k = 0; k < NB; ++k)
       for (int a = 0; a < NBC; ++a) b[k][a] = 0.0;
-    // one-sided stencils: rows RC + kb * RB + rr (a rolled loop; the target
-    // stencil's coefficients are picked with compile-time indices)
-    rw_cur = g < RB ? m.rows[(RC + g) * n + i] : int32_t(0);
 #pragma unroll 1
-    for (int kb = 0; kb < NB; ++kb) {
-      const int roff = RC + kb * RB;
-      const int off = NC * RC + kb * NBC * RB;
+    for (int r0 = R0; r0 < R; r0 += NV) {
+      double uk[NV], my_er, my_eE;
+      chunk(r0, uk, my_er, my_eE);
+#pragma unroll 1
+      for (int k = 0; k < NV; ++k) {
+        const int r = r0 + k;  // uniform over the group
+        if (r >= R) break;
+        double er = 0.0, eE = 0.0;
+        if (wb) member_eq(my_er, my_eE, k, er, eE);
+        const double d = deviation(uk[k], er, eE);
+        if (r < RC) {
 #pragma unroll
-      for (int c0 = 0; c0 < RB; c0 += NV) {
-        const int my_rr = c0 + g;
-        const bool mine = my_rr < RB;
-        double uk[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
-        double pr[NBC];
-#pragma unroll
-        for (int a = 0; a < NBC; ++a) pr[a] = mine ? m.P[(off + a * RB + my_rr) * n + i] : 0.0;
-        int32_t rw_next = 0;
-        if (my_rr + NV < RB) rw_next = m.rows[(roff + my_rr + NV) * n + i];
-        else if (c0 + NV >= RB && kb + 1 < NB && g < RB) rw_next = m.rows[(roff + RB + g) * n + i];
-        double my_er = 0.0, my_eE = 0.0;
-        if (eq_sub && mine) {
-          Member mb{};
-          load_member(rw_cur, mb);
-          member_avg_of(mb, my_er, my_eE);
-        }
-        xch.sync();
-#pragma unroll
-        for (int a = 0; a < NBC; ++a) pl[g * NC + a] = pr[a];
-        xch.sync();
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          double er = 0.0, eE = 0.0;
-          if (wb) {
-            member_eq(my_er, my_eE, k, er, eE);
-          }
-          // rows past the stencil's last (c0 + k >= RB) add exact zeros, as
-          // in the central loop
-          const double d = deviation(uk[k], er, eE);
+          for (int a = 0; a < NC; ++a) c[a] = fma(pl[k * NC + a], d, c[a]);
+        } else {
+          const int kb = (r - RC) / RB;
 #pragma unroll
           for (int a = 0; a < NBC; ++a) {
             const double p = pl[k * NC + a];
@@ -913,7 +909,6 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
               if (kk == kb) b[kk][a] = fma(p, d, b[kk][a]);
           }
         }
-        rw_cur = rw_next;
       }
     }
   }
