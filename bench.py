@@ -241,6 +241,15 @@ def main():
     value = n * 3 * args.steps / el
     log(f"setup {t_setup:.1f}s; {args.steps} steps in {el:.3f}s: {value:.3e} cell-stage updates/s")
 
+    # the drop-in RateOfChange::compute path: host state in, host tendency out
+    # (AoS <-> SoA on the host and PCIe both ways); reported beside, never value
+    dudt = wl.u0 * 0.0
+    s.rate(wl.u0, dudt=dudt)
+    t0 = time.perf_counter()
+    n_rate = 10
+    for _ in range(n_rate):
+        s.rate(wl.u0, dudt=dudt)
+    rate_ms = 1e3 * (time.perf_counter() - t0) / n_rate
     kt = s.time_kernels(max(2, min(10, args.steps)), dt)
     launches = kt.n_launches
     rec_ms = kt.reconstruct_ms / launches
@@ -293,6 +302,9 @@ def main():
                               "scaled by the copy-kernel calibration)",
         },
         "setup_s": t_setup,
+        "host_buffer_rate": {"ms_per_call": rate_ms, "cell_stage_per_s": n / (rate_ms * 1e-3),
+                             "note": "zfv_rate on host buffers (one RoC evaluation incl. PCIe and layout "
+                                     "conversion), the drop-in RateOfChange::compute path"},
         "tuning": s.tune(-1).as_dict(),
         "measured_copy_gbs": z.bandwidth_probe(1 << 30, 10),
     }
