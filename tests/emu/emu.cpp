@@ -42,7 +42,7 @@ struct LaneBarrier {
   std::atomic<long> gen{0};
   std::atomic<bool> broken{false};
   double slot[8] = {0};
-  double lds[64] = {0};  // the group's LDS slice
+  double lds[128] = {0};  // the group's LDS slice (slice_doubles <= 128)
   explicit LaneBarrier(int n_) : n(n_) {}
   void wait() {
     if (broken.load()) throw std::runtime_error("lane group broken");
@@ -154,6 +154,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.ne = im.ne;
     m.P = im.P.data();
     m.rows = im.rows.data();
+    m.twin = im.twin.data();
     m.crn = im.crn.data();
     m.xc = im.xc.data();
     m.len = im.len.data();

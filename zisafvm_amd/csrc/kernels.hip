@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(
     Img m, const double *__restrict__ U, double *__restrict__ fpv, double *__restrict__ src, int skip_far_ghosts) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
-  // one slice of NV * (DOFC - 1) doubles per cell group, sized at launch
+  // one slice of slice_doubles doubles per cell group, sized at launch
   // (rec_lds_bytes) for the block's waves
   extern __shared__ double slices[];
   const int lane = threadIdx.x & 63;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(
   // under the frozen BC a ghost cell away from the interior feeds no flux and
   // its own tendency is overwritten: nothing to reconstruct
   if (skip_far_ghosts && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * (NV * (DOFC - 1))]};
+  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
 }
 
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const Bloc
   const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + wave) * GPW + grp;
   if (grp >= GPW || i >= d.m.n) return;
   if (d.m.steady != nullptr && (d.m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * (NV * (DOFC - 1))]};
+  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()]};
   reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
 }
 
@@ -407,16 +407,16 @@ inline unsigned xcd_grid(int64_t n, int bs) { return (blocks_for(n, bs) + 7u) / 
 constexpr int kRecConfigs = 3;
 inline int rec_cfg_waves(int cfg) { return 1 << cfg; }
 // LDS of a reconstruction block of `waves` wavefronts: one slice per cell group
-template <int DIM, int DOFC>
+template <int DIM, int DOFC, int RC>
 inline unsigned rec_lds_bytes(int waves) {
-  return unsigned(waves * groups_per_wave<DIM + 2>() * (DIM + 2) * (DOFC - 1) * sizeof(double));
+  return unsigned(waves * groups_per_wave<DIM + 2>() * slice_doubles<DIM, DOFC, RC>() * sizeof(double));
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg, int skip, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
   const int cpb = waves * groups_per_wave<DIM + 2>();
-  const unsigned lds = rec_lds_bytes<DIM, DOFC>(waves);
+  const unsigned lds = rec_lds_bytes<DIM, DOFC, RC>(waves);
   hipLaunchKernelGGL((k_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(xcd_grid(m.n, cpb)), dim3(64 * waves), lds, s,
                      m, U, fpv, src, skip);
 }
@@ -424,7 +424,7 @@ void launch_rec(const Img &m, const double *U, double *fpv, double *src, int cfg
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
 void launch_brec(DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
   const int waves = rec_cfg_waves(cfg);
-  const unsigned lds = rec_lds_bytes<DIM, DOFC>(waves);
+  const unsigned lds = rec_lds_bytes<DIM, DOFC, RC>(waves);
   hipLaunchKernelGGL((kb_reconstruct<DIM, DOFC, RC, NB, RB, Q, PH>), dim3(nb), dim3(64 * waves), lds, s, d, b, stage);
 }
 
@@ -711,6 +711,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.ne = im.ne;
   m.P = h->upload(im.P);
   m.rows = h->upload(im.rows);
+  m.twin = h->upload(im.twin);
   m.crn = h->upload(im.crn);
   m.xc = h->upload(im.xc);
   m.len = h->upload(im.len);

@@ -1020,6 +1020,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   const size_t N = size_t(n);
   img.P.assign(size_t(s.m) * N, 0.0);
   img.rows.assign(size_t(s.nrows) * N, 0);
+  img.twin.assign(size_t(std::max(s.nrows - s.rc, 1)) * N, int8_t(-1));
   img.crn.assign(size_t(s.nfaces) * dim * N, 0.0);
   img.xc.assign(size_t(dim) * N, 0.0);
   img.len.assign(N, 0.0);
@@ -1038,6 +1039,12 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
       const int32_t rc = rows_c[size_t(i) * s.nrows + r];
       img.rows[size_t(r) * N + ip] = img.iperm[rc & ((1 << 27) - 1)] | (rc & ~((1 << 27) - 1));
     }
+    for (int r = s.rc; r < s.nrows; ++r)
+      for (int rp = 0; rp < s.rc && rp < 127; ++rp)
+        if (img.rows[size_t(rp) * N + ip] == img.rows[size_t(r) * N + ip]) {
+          img.twin[size_t(r - s.rc) * N + ip] = int8_t(rp);
+          break;
+        }
     for (int k = 0; k < nv; ++k)
       for (int d = 0; d < dim; ++d) img.crn[size_t(k * dim + d) * N + ip] = g.c(i, k)[d];
     for (int d = 0; d < dim; ++d) img.xc[size_t(d) * N + ip] = g.ctr[i][d];

@@ -47,6 +47,10 @@ struct HostImage {
   int64_t ne = 0;  // interior faces
   std::vector<double> P;         // m * n
   std::vector<int32_t> rows;     // nrows * n   (product numbering)
+  // one-sided row r (rc <= r < nrows) whose member (cell and translation) is
+  // also central row twin[(r - rc) * n + i]: its equilibrium average is the
+  // central row's; -1 if none
+  std::vector<int8_t> twin;      // (nrows - rc) * n
   std::vector<double> crn;       // (nfaces*dim) * n, corner k coordinate d at (k*dim+d)
   std::vector<double> xc;        // dim * n
   std::vector<double> len, vol, inr;

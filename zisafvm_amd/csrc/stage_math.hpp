@@ -46,6 +46,7 @@ struct Img {
   int64_t ne;  // interior faces
   const double *P;
   const int32_t *rows;  // cell | shift_code << 27
+  const int8_t *twin;   // (nrows - rc) * n: central row with the same member, or -1
   const double *crn;
   const double *xc;
   const double *len;
@@ -544,6 +545,12 @@ ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f)
 // Every call of bcast sits in control flow that is uniform over the group.
 //
 // U: state, nvars x n SoA.  fpv: [side][q][v] x ne.  src: nvars x n or null.
+// LDS doubles of one cell group: the NV pseudo-inverse rows of the chunk
+// being accumulated, then the central rows' member equilibrium averages
+// (er, eE per row) for the one-sided rows that share their member
+template <int DIM, int DOFC, int RC>
+constexpr int slice_doubles() { return (DIM + 2) * (DOFC - 1) + 2 * RC; }
+
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, class X>
 ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const double *__restrict__ U,
                              double *__restrict__ fpv, double *__restrict__ src) {
@@ -775,7 +782,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // entries are loaded one chunk ahead, so memory latency overlaps
     // arithmetic.  Central rows run in a rolled loop; the one-sided stencils
     // are unrolled so that their coefficients stay in registers.
-    double *pl = xch.lds();  // NV rows x NC coefficients
+    double *pl = xch.lds();       // NV rows x NC coefficients
+    double *ceq = pl + NV * NC;   // central rows' member averages (er, eE)
     // scaled deviation of row k of the chunk in this lane's variable
     // (the equilibrium parts enter with factors 1 or 0: u - 1 er = u - er
     // and x - 0 eE = x exactly, so this equals the branchy form bit for bit)
@@ -842,12 +850,33 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       const bool mine = my_r < R;
 #pragma unroll
       for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
+      // a one-sided row whose member is a central row of an earlier chunk
+      // takes that row's averages from LDS (the same values: same member,
+      // same function) instead of loading the member and averaging again;
+      // not in PH_LIN, whose member averages cost less than the lookup
+      auto twin_before = [&](int r, int chunk0) {
+        if (PH == PH_LIN || !eq_sub || r < RC || r >= R) return -1;
+        const int t = m.twin[(r - RC) * n + i];
+        return t < chunk0 ? t : -1;
+      };
+      const int tw = twin_before(my_r, r0);
       Member mb_nxt{};
-      if (eq_sub && my_r + NV < R) load_member(rw_nxt, mb_nxt);
+      if (eq_sub && my_r + NV < R && twin_before(my_r + NV, r0 + NV) < 0) load_member(rw_nxt, mb_nxt);
       const int32_t rw_far = my_r + 2 * NV < R ? m.rows[(my_r + 2 * NV) * n + i] : int32_t(0);
       my_er = 0.0;
       my_eE = 0.0;
-      if (eq_sub && mine) member_avg_of(mb_cur, my_er, my_eE);
+      if (eq_sub && mine) {
+        if (tw >= 0) {
+          my_er = ceq[2 * tw];
+          my_eE = ceq[2 * tw + 1];
+        } else {
+          member_avg_of(mb_cur, my_er, my_eE);
+        }
+        if (PH != PH_LIN && my_r < RC) {
+          ceq[2 * my_r] = my_er;
+          ceq[2 * my_r + 1] = my_eE;
+        }
+      }
       double pr[NC];
       if (my_r < RC) {
 #pragma unroll
