@@ -54,8 +54,8 @@ def test_translation_leaves_the_rate_unchanged(gpu):
                      wl.mesh.ghost)
     s2 = z.Solver(shifted, wl.params)
     L1 = s2.rate(wl.u0)
-    den = np.abs(L0).max(axis=0)
-    err = (np.abs(L1 - L0).max(axis=0) / den)[[0, 1, 2, 4]]
+    cols = [0, 1, 2, 4]  # m_z is zero in 2D
+    err = np.abs(L1 - L0)[:, cols].max(axis=0) / np.abs(L0)[:, cols].max(axis=0)
     print("translation: rel change per variable", err)
     assert err.max() <= 1e-10
 
