@@ -775,13 +775,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 
   if (!first_order) {
     // Rows in chunks of NV: lane k of the group owns row r0 + k of a chunk.
-    // It loads that row's member (corners and volume for the equilibrium
-    // average) and the row of the pseudo-inverse, which reaches every lane
-    // through the group's LDS slice.  A chunk issues all its loads before the
-    // member's equilibrium average is computed, and the next chunk's row
-    // entries are loaded one chunk ahead, so memory latency overlaps
-    // arithmetic.  Central rows run in a rolled loop; the one-sided stencils
-    // are unrolled so that their coefficients stay in registers.
+    // It computes that row's member equilibrium average (member data loaded
+    // one chunk ahead, row entries two chunks ahead, so memory latency
+    // overlaps arithmetic), then loads the row of the pseudo-inverse, which
+    // reaches every lane through the group's LDS slice; the averages reach
+    // the lanes by wavefront shuffles.
     double *pl = xch.lds();       // NV rows x NC coefficients
     double *ceq = pl + NV * NC;   // central rows' member averages (er, eE)
     // scaled deviation of row k of the chunk in this lane's variable
@@ -801,8 +799,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       else eE = xch.bcast(my_eE, k);
     };
 
-    // translated corners and volume of a member (row entry rw)
-    // what the equilibrium average over a member needs: its potential
+    // what the equilibrium average over a member (row entry rw) needs: its potential
     // moments (linear isentropic equilibrium, untranslated member) or its
     // translated corners and volume
     struct Member {
