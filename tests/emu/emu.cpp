@@ -28,7 +28,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim, pot;
+  std::vector<double> U, fpv, G, src, phim, pot, frm;
 };
 
 // The lanes of one cell's group run as NV host threads in lockstep: every
@@ -167,6 +167,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.steady = nullptr;
     m.phim = nullptr;
     m.pot = nullptr;
+    m.frm = nullptr;
     m.gamma = p->gamma;
     m.eps = p->epsilon;
     m.exponent = p->exponent;
@@ -203,6 +204,15 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
       }
       m.phim = e->phim.data();
     }
+    // the face frame tables the GPU path keeps
+    e->frm.assign(size_t(im.s.dim == 2 ? 3 * frame_words<2>() : 4 * frame_words<3>()) * im.n, 0.0);
+    for (int64_t i = 0; i < im.n; ++i) {
+      if (im.s.dim == 2)
+        store_face_frames<2>(m, i, e->frm.data());
+      else
+        store_face_frames<3>(m, i, e->frm.data());
+    }
+    m.frm = e->frm.data();
     if (physics_set(m) != PH_LIGHT && p->gravity != ZFV_GRAVITY_NONE) {
       // the potential tables the GPU path keeps for these physics sets
       const int rows = im.s.dim == 2 ? pot_rows<2>(m.qv, im.s.q) : pot_rows<3>(m.qv, im.s.q);

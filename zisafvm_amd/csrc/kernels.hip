@@ -181,6 +181,14 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// per-cell face frame tables
+template <int DIM>
+__global__ __launch_bounds__(256) void k_face_frames(Img m, double *tab) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  store_face_frames<DIM>(m, i, tab);
+}
+
 // per-cell potential tables of the cells' own quadrature points
 template <int DIM>
 __global__ __launch_bounds__(256) void k_potential_points(Img m, int nqf, double *tab) {
@@ -724,6 +732,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.steady = nullptr;
   m.phim = nullptr;
   m.pot = nullptr;
+  m.frm = nullptr;
   m.gamma = params->gamma;
   m.eps = params->epsilon;
   m.exponent = params->exponent;
@@ -770,6 +779,17 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(h->stream));
     m.phim = pm;
+  }
+  {
+    // face frames of every cell's local faces
+    double *tab = h->alloc<double>(size_t(s.dim == 2 ? 3 * frame_words<2>() : 4 * frame_words<3>()) * n);
+    if (s.dim == 2)
+      hipLaunchKernelGGL(k_face_frames<2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, tab);
+    else
+      hipLaunchKernelGGL(k_face_frames<3>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, tab);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    m.frm = tab;
   }
   h->ph = physics_set(m);
   if (h->ph != PH_LIGHT && params->gravity != ZFV_GRAVITY_NONE) {
@@ -925,14 +945,15 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   // instance has them, the face-point values written once per interior face
   // side and the source term written (with gravity)
   const double nv = s.nvars;
+  const double frame = s.dim == 2 ? frame_words<2>() : frame_words<3>();
   const Img &dm = h->dimg;
   const double tables = (dm.pot ? 8.0 * (s.dim == 2 ? pot_rows<2>(dm.qv, s.q) : pot_rows<3>(dm.qv, s.q)) : 0.0) +
                         (dm.phim ? 16.0 : 0.0);
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
-                     4.0 * (1 + s.nfaces + 1) + tables + 8.0 * (2.0 * ne / n) * s.q * nv +
-                     (h->src ? 8.0 * nv : 0.0);
+                     4.0 * (1 + s.nfaces + 1) + tables +
+                     8.0 * (2.0 * ne / n) * s.q * nv + (h->src ? 8.0 * nv : 0.0);
   const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv);
-  const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * s.dim + 1) + 4.0 + 8.0 * nv * 5;
+  const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * frame + 1) + 4.0 + 8.0 * nv * 5;
   info->recon_bytes_per_cell = rec;
   info->bytes_per_cell_stage = rec + face + upd;
   int maxs = 0;
@@ -1026,9 +1047,16 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   const double n = mesh->n_cells;
   const double ne = n * (s.nfaces / 2.0);  // interior faces <= cells * faces / 2
   const double nv = s.nvars;
-  double per_cell = 8.0 * s.m + 4.0 * s.nrows + 8.0 * (s.nfaces * s.dim + s.dim + 3 + s.nmom) + 4.0 * (2 + s.nfaces) +
-                    8.0 * nv * (1 + 1 + 3);  // U, X, K[3]
-  if (params->gravity != ZFV_GRAVITY_NONE) per_cell += 8.0 * nv;       // source
+  const double frame = s.dim == 2 ? frame_words<2>() : frame_words<3>();
+  double per_cell = 8.0 * s.m + 4.0 * s.nrows + 1.0 * (s.nrows - s.rc) +
+                    8.0 * (s.nfaces * s.dim + s.dim + 3 + s.nmom) + 4.0 * (2 + s.nfaces) +
+                    8.0 * s.nfaces * frame +    // face frames
+                    8.0 * nv * (1 + 1 + 3);     // U, X, K[3]
+  if (params->gravity != ZFV_GRAVITY_NONE) {
+    per_cell += 8.0 * nv;                       // source
+    per_cell += 8.0 * 2;                        // potential moments
+    per_cell += 8.0 * (16 * (1 + s.dim) + s.nfaces * s.q);  // potential tables (at most)
+  }
   if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
   double per_face = 8.0 * nv * (2.0 * s.q + 1.0);                      // point values + fluxes
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);

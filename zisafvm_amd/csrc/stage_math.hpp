@@ -59,6 +59,7 @@ struct Img {
   const double *steady;  // nvars * n (ghost cells only are read)
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
+  const double *frm;     // face frames of the cells' local faces (load_face_frame)
   // physics
   double gamma;
   double eps, exponent;
@@ -309,6 +310,62 @@ ZFV_HD void face_frame(const double (*w)[3], double *n, double *t1, double *t2, 
     double sc = la;
     double s = 0.5 * (sa + sb + sc);
     area = sqrt(s * (s - sa) * (s - sb) * (s - sc));
+  }
+  t2[0] = n[1] * t1[2] - n[2] * t1[1];
+  t2[1] = n[2] * t1[0] - n[0] * t1[2];
+  t2[2] = n[0] * t1[1] - n[1] * t1[0];
+}
+
+// Face frames are time independent: each cell's local faces keep theirs in a
+// table written once at setup (store_face_frames, with face_frame itself), so
+// the cell update reads 3 (2D: t1, area) or 7 (3D: n, t1, area) doubles per
+// face instead of loading the corners and rebuilding frames; n (2D) and t2
+// follow from the stored values with face_frame's own formulas, so every
+// value equals face_frame's.  (The reconstruction, which loads the face
+// vertices for its quadrature points anyway, rebuilds them: measured faster.)
+template <int DIM>
+constexpr int frame_words() { return DIM == 2 ? 3 : 7; }
+
+// writes the frames of cell i's local faces (tab has frame_words (d+1) n entries)
+template <int DIM>
+ZFV_HD void store_face_frames(const Img &m, int64_t i, double *tab) {
+  constexpr int W = frame_words<DIM>();
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+  const uint32_t fperm = m.fperm[i];
+  for (int f = 0; f < DIM + 1; ++f) {
+    double w3[3][3], n[3], t1[3], t2[3], area;
+    face_vertices<DIM>(fperm, f, vtx, w3);
+    face_frame<DIM>(w3, n, t1, t2, area);
+    double *p = tab + int64_t(f * W) * m.n + i;
+    if (DIM == 2) {
+      p[0] = t1[0];
+      p[m.n] = t1[1];
+      p[2 * m.n] = area;
+    } else {
+      for (int d = 0; d < 3; ++d) p[int64_t(d) * m.n] = n[d];
+      for (int d = 0; d < 3; ++d) p[int64_t(3 + d) * m.n] = t1[d];
+      p[int64_t(6) * m.n] = area;
+    }
+  }
+}
+
+template <int DIM>
+ZFV_HD void load_face_frame(const Img &m, int64_t i, int f, double *n, double *t1, double *t2, double &area) {
+  constexpr int W = frame_words<DIM>();
+  const double *p = m.frm + int64_t(f * W) * m.n + i;
+  if (DIM == 2) {
+    t1[0] = p[0];
+    t1[1] = p[m.n];
+    t1[2] = 0.0;
+    area = p[2 * m.n];
+    n[0] = t1[1];
+    n[1] = -t1[0];
+    n[2] = t1[2];
+  } else {
+    for (int d = 0; d < 3; ++d) n[d] = p[int64_t(d) * m.n];
+    for (int d = 0; d < 3; ++d) t1[d] = p[int64_t(3 + d) * m.n];
+    area = p[int64_t(6) * m.n];
   }
   t2[0] = n[1] * t1[2] - n[2] * t1[1];
   t2[1] = n[2] * t1[0] - n[0] * t1[2];
@@ -1039,7 +1096,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     double w3[3][3];
     load_face_vertices<DIM>(m, i, fperm, f, w3);
     double nn[3], t1[3], t2[3], area;
-    face_frame<DIM>(w3, nn, t1, t2, area);
+    face_frame<DIM>(w3, nn, t1, t2, area);  // (the vertices are needed here anyway)
     const bool exterior = (cf & 2) != 0;
     // outward normal sign for the source term (src/zisa/math/face.cpp:24-27)
     double x0f[3];
@@ -1186,9 +1243,6 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
   constexpr int NV = DIM + 2;
   constexpr int NF = DIM + 1;
   const int64_t n = m.n;
-  double vtx[DIM + 1][3];
-  load_corners<DIM>(m, i, vtx);
-  const uint32_t fperm = m.fperm[i];
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 #pragma unroll
@@ -1197,10 +1251,8 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
     const bool exterior = (cf & 2) != 0;
     // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC
     if (exterior && (m.flux_bc == 0 || (cf & 1))) continue;
-    double w3[3][3];
-    face_vertices<DIM>(fperm, f, vtx, w3);
     double nn[3], t1[3], t2[3], area;
-    face_frame<DIM>(w3, nn, t1, t2, area);
+    load_face_frame<DIM>(m, i, f, nn, t1, t2, area);
     double g[NV];
     double sgn;
     if (!exterior) {
