@@ -250,6 +250,12 @@ def main():
     for _ in range(n_rate):
         s.rate(wl.u0, dudt=dudt)
     rate_ms = 1e3 * (time.perf_counter() - t0) / n_rate
+    # the device time loop (zfv_advance): LocalCFL chosen on the device every
+    # step, dt clamped to the end time, no host round trip per step
+    s.synchronize()
+    t0 = time.perf_counter()
+    _, adv_steps = s.advance(wl.cfl, 0.0, 1.0e9, max_steps=args.steps, check_every=args.steps)
+    el_adv = time.perf_counter() - t0
     kt = s.time_kernels(max(2, min(10, args.steps)), dt)
     launches = kt.n_launches
     rec_ms = kt.reconstruct_ms / launches
@@ -302,6 +308,9 @@ def main():
                               "scaled by the copy-kernel calibration)",
         },
         "setup_s": t_setup,
+        "device_time_loop": {"steps": adv_steps, "ms_per_step": 1e3 * el_adv / max(adv_steps, 1),
+                             "cell_stage_per_s": n * 3 * adv_steps / el_adv,
+                             "note": "zfv_advance: CFL dt chosen on the device every step"},
         "host_buffer_rate": {"ms_per_call": rate_ms, "cell_stage_per_s": n / (rate_ms * 1e-3),
                              "note": "zfv_rate on host buffers (one RoC evaluation incl. PCIe and layout "
                                      "conversion), the drop-in RateOfChange::compute path"},

@@ -125,6 +125,16 @@ int zfv_step(zfv_handle *h, double t, double dt);
 /* n_steps fixed-dt SSP3 steps replayed from one captured HIP graph. */
 int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt);
 
+/* TimeLoop's inner loop on the device (src/zisa/fvm_loops/time_loop.cpp:36-81):
+ * from t0, SSP3 steps with dt = cfl_number * LocalCFL (computed on the device,
+ * include/zisa/model/local_cfl_condition_impl.hpp:25-40) clamped to t_end,
+ * until t_end or max_steps (<= 0: no limit).  One captured graph per step;
+ * the host reads the loop state back every check_every steps (< 1: 16), so a
+ * finished run may replay up to check_every - 1 steps of dt 0 (no change).
+ * Writes the final time and the number of steps taken. */
+int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps, int32_t check_every,
+                double *t_final, int32_t *steps_done);
+
 /* LocalCFL::operator() on the device-resident state
  * (include/zisa/model/local_cfl_condition_impl.hpp:25-40). */
 int zfv_cfl_dt(zfv_handle *h, double cfl_number, double *dt);

@@ -201,6 +201,7 @@ def load_library(path: str = LIB_PATH):
         "zfv_step": (c_int, [vp, c_double, c_double]),
         "zfv_steps": (c_int, [vp, ctypes.c_int32, c_double, c_double]),
         "zfv_cfl_dt": (c_int, [vp, c_double, dp]),
+        "zfv_advance": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, dp, ip]),
         "zfv_synchronize": (c_int, [vp]),
         "zfv_get_info": (c_int, [vp, ctypes.POINTER(Info)]),
         "zfv_get_stencils": (c_int, [vp, ctypes.c_int32, ip, ctypes.c_int32, ip, ip, ctypes.c_int32]),
@@ -372,6 +373,14 @@ class Solver:
 
     def steps(self, n: int, t: float, dt: float):
         _check(self.lib.zfv_steps(self.h, n, t, dt))
+
+    def advance(self, cfl: float, t0: float, t_end: float, max_steps: int = 0, check_every: int = 16):
+        """TimeLoop on the device: LocalCFL steps from t0 to t_end (zfv_advance);
+        returns (t_final, steps)."""
+        t = ctypes.c_double(0.0)
+        k = ctypes.c_int32(0)
+        _check(self.lib.zfv_advance(self.h, cfl, t0, t_end, max_steps, check_every, ctypes.byref(t), ctypes.byref(k)))
+        return t.value, k.value
 
     def cfl_dt(self, cfl: float) -> float:
         dt = ctypes.c_double()

@@ -110,11 +110,12 @@ struct StageCoeffs {
 // k_update: K[stage] = L(Ustage); Unext = U0 + dt * sum_j c_j K[j]
 // (src/zisa/ode/runge_kutta.cpp:122-143), then FrozenBC for ghost cells.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage, double dt,
+__global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage, double dt, const double *dtp,
                                                 const double *__restrict__ Ustage, const double *__restrict__ G,
                                                 const double *__restrict__ src, const double *U0,
                                                 double *__restrict__ K, double *Unext) {
   constexpr int NV = DIM + 2;
+  if (dtp) dt = *dtp;  // the step's dt chosen on the device (zfv_advance)
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int64_t n = m.n;
   if (i >= n) return;
@@ -368,6 +369,45 @@ __global__ void kb_choose_dt(DevInst *insts, int n_inst) {
   if (!(dt > 0.0)) d.active = 0;
 }
 
+// A single instance's time loop on the device (zfv_advance): TimeLoop's
+// step with LocalCFL (src/zisa/fvm_loops/time_loop.cpp:36-81,
+// include/zisa/model/local_cfl_condition_impl.hpp:25-40), dt clamped to the
+// end time as the simulation clock does (its library is not vendored in the
+// reference), with no host round trip per step.
+struct AdvState {
+  double t, t_end, cfl, dt;
+  unsigned long long dt_bits;
+  int steps, max_steps, active, pad;
+};
+
+__global__ void k_adv_begin(AdvState *a) {
+  const double big = 1.0e300;
+  unsigned long long bits;
+  memcpy(&bits, &big, sizeof(bits));
+  a->dt_bits = bits;
+}
+
+__global__ void k_adv_choose(AdvState *a) {
+  double m;
+  unsigned long long bits = a->dt_bits;
+  memcpy(&m, &bits, sizeof(m));
+  double dt = a->cfl * m;
+  const double rest = a->t_end - a->t;
+  if (dt > rest) dt = rest;
+  if (!a->active || !(dt > 0.0)) {
+    dt = 0.0;  // a finished run's stages leave U as it is (U + 0 * dudt)
+    a->active = 0;
+  }
+  a->dt = dt;
+}
+
+__global__ void k_adv_end(AdvState *a) {
+  if (!a->active) return;
+  a->t += a->dt;
+  a->steps += 1;
+  if (a->t >= a->t_end || a->steps >= a->max_steps) a->active = 0;
+}
+
 __global__ void kb_end_step(DevInst *insts, int n_inst) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_inst) return;
@@ -394,7 +434,7 @@ struct Launcher {
   void (*rec[3])(const Img &, const double *, double *, double *, int cfg, int skip_far_ghosts, hipStream_t);
   void (*face)(const Img &, const double *, double *, hipStream_t);
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
-                 const double *, double *, double *, hipStream_t);
+                 const double *, const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
   // batched variants: grid = number of block-table entries
   void (*brec[3])(DevInst *, const BlockRef *, unsigned, int, int cfg, hipStream_t);
@@ -452,10 +492,10 @@ Launcher make_launcher() {
     if (m.ne > 0)
       hipLaunchKernelGGL((k_face_flux<DIM, Q>), dim3(xcd_grid(m.ne, 256)), dim3(256), 0, s, m, fpv, G);
   };
-  l.update = [](const Img &m, StageCoeffs sc, int stage, double dt, const double *Us, const double *G,
-                const double *src, const double *U0, double *K, double *Un, hipStream_t s) {
-    hipLaunchKernelGGL((k_update<DIM>), dim3(xcd_grid(m.n, 256)), dim3(256), 0, s, m, sc, stage, dt, Us, G, src,
-                       U0, K, Un);
+  l.update = [](const Img &m, StageCoeffs sc, int stage, double dt, const double *dtp, const double *Us,
+                const double *G, const double *src, const double *U0, double *K, double *Un, hipStream_t s) {
+    hipLaunchKernelGGL((k_update<DIM>), dim3(xcd_grid(m.n, 256)), dim3(256), 0, s, m, sc, stage, dt, dtp, Us, G,
+                       src, U0, K, Un);
   };
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
     hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
@@ -524,6 +564,8 @@ struct zfv_handle {
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = -1.0;
   int graph_steps = 0;
+  zfv::AdvState *adv = nullptr;           // device time-loop state (zfv_advance)
+  hipGraphExec_t adv_graph = nullptr;     // one adaptive step
   int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
   double rec_ms[zfv::kRecConfigs] = {0, 0, 0};
   int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL / PH_LIN)
@@ -545,6 +587,7 @@ struct zfv_handle {
   }
   ~zfv_handle() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (adv_graph) (void)hipGraphExecDestroy(adv_graph);
     for (void *p : allocations) (void)hipFree(p);
     if (host_stage) (void)hipHostFree(host_stage);
     if (stream) (void)hipStreamDestroy(stream);
@@ -648,9 +691,34 @@ static void enqueue_step(zfv_handle *h, double dt) {
     h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, frozen(h), h->stream);
     h->L.face(m, h->fpv, h->G, h->stream);
     double *Un = (s == 2) ? h->U : h->X;
-    h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+    h->L.update(m, ssp3_coeffs(s), s, dt, nullptr, Us, h->G, h->src, h->U, h->K, Un, h->stream);
     Us = h->X;
   }
+}
+
+// captured step graphs, rebuilt on next use
+static void drop_graphs(zfv_handle *h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->adv_graph) (void)hipGraphExecDestroy(h->adv_graph);
+  h->graph_exec = nullptr;
+  h->adv_graph = nullptr;
+}
+
+// one step with the dt the device chooses (LocalCFL, clamped to t_end)
+static void enqueue_adaptive_step(zfv_handle *h) {
+  const Img &m = h->dimg;
+  hipLaunchKernelGGL(k_adv_begin, dim3(1), dim3(1), 0, h->stream, h->adv);
+  h->L.cfl(m, h->U, &h->adv->dt_bits, h->stream);
+  hipLaunchKernelGGL(k_adv_choose, dim3(1), dim3(1), 0, h->stream, h->adv);
+  const double *Us = h->U;
+  for (int s = 0; s < 3; ++s) {
+    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, frozen(h), h->stream);
+    h->L.face(m, h->fpv, h->G, h->stream);
+    double *Un = (s == 2) ? h->U : h->X;
+    h->L.update(m, ssp3_coeffs(s), s, 0.0, &h->adv->dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+    Us = h->X;
+  }
+  hipLaunchKernelGGL(k_adv_end, dim3(1), dim3(1), 0, h->stream, h->adv);
 }
 
 static void to_soa(const zfv_handle *h, const double *aos, double *soa) {
@@ -843,10 +911,7 @@ int zfv_set_steady_state(zfv_handle *h, const double *cvars) {
   HIP_CHECK(hipMemcpyAsync(h->steady, h->host_stage, sizeof(double) * nv * n, hipMemcpyHostToDevice, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   h->dimg.steady = h->steady;
-  if (h->graph_exec) {
-    (void)hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
-  }
+  drop_graphs(h);  // the captured steps hold the BC and the skip flag
   return ZFV_OK;
   ZFV_API_END
 }
@@ -863,7 +928,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
   h->L.face(m, h->fpv, h->G, h->stream);
   StageCoeffs sc{};
   sc.write_rate_only = 1;
-  h->L.update(m, sc, 0, 0.0, h->X, h->G, h->src, h->X, h->K, h->X, h->stream);
+  h->L.update(m, sc, 0, 0.0, nullptr, h->X, h->G, h->src, h->X, h->K, h->X, h->stream);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(h->host_stage, h->K, sizeof(double) * nv * n, hipMemcpyDeviceToHost, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -903,6 +968,41 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
   }
   for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(h->graph_exec, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps, int32_t check_every,
+                double *t_final, int32_t *steps_done) {
+  ZFV_API_BEGIN
+  if (!(cfl_number > 0.0)) throw ZfvError(ZFV_ERR_INVALID, "cfl_number must be > 0");
+  if (check_every < 1) check_every = 16;
+  rec_config(h, h->U);
+  if (!h->adv) h->adv = h->alloc<AdvState>(1);
+  if (!h->adv_graph) {
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_adaptive_step(h);
+    HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
+    HIP_CHECK(hipGraphInstantiate(&h->adv_graph, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+  }
+  AdvState a{};
+  a.t = t0;
+  a.t_end = t_end;
+  a.cfl = cfl_number;
+  a.steps = 0;
+  a.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
+  a.active = t0 < t_end ? 1 : 0;
+  HIP_CHECK(hipMemcpyAsync(h->adv, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  while (a.active) {
+    for (int k = 0; k < check_every; ++k) HIP_CHECK(hipGraphLaunch(h->adv_graph, h->stream));
+    HIP_CHECK(hipMemcpyAsync(&a, h->adv, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+  }
+  if (t_final) *t_final = a.t;
+  if (steps_done) *steps_done = a.steps;
   return ZFV_OK;
   ZFV_API_END
 }
@@ -975,10 +1075,7 @@ int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out) {
   } else if (rec_threads > 0) {
     h->rec_cfg = rec_cfg_of(rec_threads);
   }
-  if (h->graph_exec && rec_threads >= 0) {  // the captured step holds the old configuration
-    HIP_CHECK(hipGraphExecDestroy(h->graph_exec));
-    h->graph_exec = nullptr;
-  }
+  if (rec_threads >= 0) drop_graphs(h);  // the captured steps hold the old configuration
   report_tuning(h->rec_cfg, h->L.groups_per_wave, h->rec_ms, out);
   return ZFV_OK;
   ZFV_API_END
@@ -1017,7 +1114,7 @@ int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       double *Un = (s == 2) ? h->U : h->X;
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
-      h->L.update(m, ssp3_coeffs(s), s, dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+      h->L.update(m, ssp3_coeffs(s), s, dt, nullptr, Us, h->G, h->src, h->U, h->K, Un, h->stream);
       HIP_CHECK(hipEventRecord(ev[k++], h->stream));
       Us = h->X;
     }
