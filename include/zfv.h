@@ -128,7 +128,8 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt);
 /* TimeLoop's inner loop on the device (src/zisa/fvm_loops/time_loop.cpp:36-81):
  * from t0, SSP3 steps with dt = cfl_number * LocalCFL (computed on the device,
  * include/zisa/model/local_cfl_condition_impl.hpp:25-40) clamped to t_end,
- * until t_end or max_steps (<= 0: no limit).  One captured graph per step;
+ * until t_end or max_steps (<= 0: no limit), or until a step would not advance
+ * the time (dt not positive or below the resolution of t).  One captured graph per step;
  * the host reads the loop state back every check_every steps (< 1: 16), so a
  * finished run may replay up to check_every - 1 steps of dt 0 (no change).
  * Writes the final time and the number of steps taken. */

@@ -394,7 +394,9 @@ __global__ void k_adv_choose(AdvState *a) {
   double dt = a->cfl * m;
   const double rest = a->t_end - a->t;
   if (dt > rest) dt = rest;
-  if (!a->active || !(dt > 0.0)) {
+  // stop when finished, when the CFL gives no positive step (a non-physical
+  // state) or when the step no longer advances the time
+  if (!a->active || !(dt > 0.0) || a->t + dt == a->t) {
     dt = 0.0;  // a finished run's stages leave U as it is (U + 0 * dudt)
     a->active = 0;
   }
