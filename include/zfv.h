@@ -132,9 +132,18 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt);
  * the time (dt not positive or below the resolution of t).  One captured graph per step;
  * the host reads the loop state back every check_every steps (< 1: 16), so a
  * finished run may replay up to check_every - 1 steps of dt 0 (no change).
- * Writes the final time and the number of steps taken. */
+ * drho_crit_rel > 0 turns on RejectLargeDensityChange
+ * (src/zisa/ode/step_rejection.cpp:17-57): a step that changes the density of
+ * any cell by more than drho_crit_rel times its old value is undone and retried
+ * with half the factor times its dt (TimeLoop::reject_step,
+ * src/zisa/fvm_loops/time_loop.cpp:182-196); the factor doubles after two good
+ * steps in a row and stays within [2^-max_exponent, 1].  It starts at 1 on every
+ * call.  drho_crit_rel <= 0 rejects nothing (RejectNothing).
+ * Writes the final time, the number of accepted steps and the number of
+ * rejected steps (each pointer may be null). */
 int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps, int32_t check_every,
-                double *t_final, int32_t *steps_done);
+                double drho_crit_rel, int32_t max_exponent, double *t_final, int32_t *steps_done,
+                int32_t *rejected);
 
 /* LocalCFL::operator() on the device-resident state
  * (include/zisa/model/local_cfl_condition_impl.hpp:25-40). */

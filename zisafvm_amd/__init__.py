@@ -201,7 +201,8 @@ def load_library(path: str = LIB_PATH):
         "zfv_step": (c_int, [vp, c_double, c_double]),
         "zfv_steps": (c_int, [vp, ctypes.c_int32, c_double, c_double]),
         "zfv_cfl_dt": (c_int, [vp, c_double, dp]),
-        "zfv_advance": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, dp, ip]),
+        "zfv_advance": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, c_double,
+                                ctypes.c_int32, dp, ip, ip]),
         "zfv_synchronize": (c_int, [vp]),
         "zfv_get_info": (c_int, [vp, ctypes.POINTER(Info)]),
         "zfv_get_stencils": (c_int, [vp, ctypes.c_int32, ip, ctypes.c_int32, ip, ip, ctypes.c_int32]),
@@ -374,12 +375,18 @@ class Solver:
     def steps(self, n: int, t: float, dt: float):
         _check(self.lib.zfv_steps(self.h, n, t, dt))
 
-    def advance(self, cfl: float, t0: float, t_end: float, max_steps: int = 0, check_every: int = 16):
+    def advance(self, cfl: float, t0: float, t_end: float, max_steps: int = 0, check_every: int = 16,
+                drho_crit_rel: float = 0.0, max_exponent: int = 0):
         """TimeLoop on the device: LocalCFL steps from t0 to t_end (zfv_advance);
-        returns (t_final, steps)."""
+        returns (t_final, accepted steps).  drho_crit_rel > 0 rejects and
+        retries steps as RejectLargeDensityChange(drho_crit_rel, max_exponent)
+        does; the number of rejected steps is kept in self.rejected_steps."""
         t = ctypes.c_double(0.0)
         k = ctypes.c_int32(0)
-        _check(self.lib.zfv_advance(self.h, cfl, t0, t_end, max_steps, check_every, ctypes.byref(t), ctypes.byref(k)))
+        r = ctypes.c_int32(0)
+        _check(self.lib.zfv_advance(self.h, cfl, t0, t_end, max_steps, check_every, drho_crit_rel, max_exponent,
+                                    ctypes.byref(t), ctypes.byref(k), ctypes.byref(r)))
+        self.rejected_steps = r.value
         return t.value, k.value
 
     def cfl_dt(self, cfl: float) -> float:

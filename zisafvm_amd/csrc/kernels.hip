@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -378,6 +379,13 @@ struct AdvState {
   double t, t_end, cfl, dt;
   unsigned long long dt_bits;
   int steps, max_steps, active, pad;
+  // RejectLargeDensityChange (src/zisa/ode/step_rejection.cpp:17-57; off
+  // when drho_crit <= 0): factor scales the CFL step, halves on a rejected
+  // step (which is retried with factor * its dt, TimeLoop::reject_step,
+  // src/zisa/fvm_loops/time_loop.cpp:182-196) and doubles after two good
+  // steps in a row, within [min_factor, 1]
+  double drho_crit, factor, min_factor, retry_dt;
+  int bad, prev_good, retry, rejections;
 };
 
 __global__ void k_adv_begin(AdvState *a) {
@@ -385,13 +393,39 @@ __global__ void k_adv_begin(AdvState *a) {
   unsigned long long bits;
   memcpy(&bits, &big, sizeof(bits));
   a->dt_bits = bits;
+  a->bad = 0;
+}
+
+// keeps u0 for a step that may be rejected (only with step rejection on)
+__global__ __launch_bounds__(256) void k_adv_backup(const AdvState *a, int64_t count, const double *U,
+                                                    double *Ubk) {
+  if (!(a->drho_crit > 0.0) || !a->active) return;
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < count) Ubk[k] = U[k];
+}
+
+// RejectLargeDensityChange::check: |rho1 - rho0| > drho_crit rho0 in any cell
+__global__ __launch_bounds__(256) void k_adv_check(AdvState *a, int64_t n, const double *U, const double *Ubk) {
+  if (!(a->drho_crit > 0.0) || !a->active) return;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (fabs(Ubk[i] - U[i]) > a->drho_crit * Ubk[i]) atomicOr(&a->bad, 1);
+}
+
+// a rejected step leaves the state as it was
+__global__ __launch_bounds__(256) void k_adv_restore(const AdvState *a, int64_t count, double *U,
+                                                     const double *Ubk) {
+  if (!a->bad || !a->active) return;
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < count) U[k] = Ubk[k];
 }
 
 __global__ void k_adv_choose(AdvState *a) {
   double m;
   unsigned long long bits = a->dt_bits;
   memcpy(&m, &bits, sizeof(m));
-  double dt = a->cfl * m;
+  // StepRejection::pick_time_step: factor * the CFL step, or the retry step
+  double dt = a->retry ? a->retry_dt : a->factor * (a->cfl * m);
   const double rest = a->t_end - a->t;
   if (dt > rest) dt = rest;
   // stop when finished, when the CFL gives no positive step (a non-physical
@@ -405,6 +439,20 @@ __global__ void k_adv_choose(AdvState *a) {
 
 __global__ void k_adv_end(AdvState *a) {
   if (!a->active) return;
+  a->retry = 0;
+  if (a->drho_crit > 0.0) {
+    const double f = a->factor;
+    if (a->bad) {
+      a->prev_good = 0;
+      a->factor = fmax(a->min_factor, fmin(f / 2.0, 1.0));
+      a->retry = 1;
+      a->retry_dt = a->factor * a->dt;
+      a->rejections += 1;
+      return;  // the time does not advance; the state was restored
+    }
+    a->factor = fmax(a->min_factor, fmin(a->prev_good ? f * 2.0 : f, 1.0));
+    a->prev_good = 1;
+  }
   a->t += a->dt;
   a->steps += 1;
   if (a->t >= a->t_end || a->steps >= a->max_steps) a->active = 0;
@@ -567,7 +615,8 @@ struct zfv_handle {
   double graph_dt = -1.0;
   int graph_steps = 0;
   zfv::AdvState *adv = nullptr;           // device time-loop state (zfv_advance)
-  hipGraphExec_t adv_graph = nullptr;     // one adaptive step
+  double *Ubk = nullptr;                  // u0 of a step that may be rejected
+  hipGraphExec_t adv_graph[2] = {nullptr, nullptr};  // one adaptive step, without / with step rejection
   int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
   double rec_ms[zfv::kRecConfigs] = {0, 0, 0};
   int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL / PH_LIN)
@@ -589,7 +638,8 @@ struct zfv_handle {
   }
   ~zfv_handle() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (adv_graph) (void)hipGraphExecDestroy(adv_graph);
+    for (hipGraphExec_t g : adv_graph)
+      if (g) (void)hipGraphExecDestroy(g);
     for (void *p : allocations) (void)hipFree(p);
     if (host_stage) (void)hipHostFree(host_stage);
     if (stream) (void)hipStreamDestroy(stream);
@@ -701,15 +751,22 @@ static void enqueue_step(zfv_handle *h, double dt) {
 // captured step graphs, rebuilt on next use
 static void drop_graphs(zfv_handle *h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-  if (h->adv_graph) (void)hipGraphExecDestroy(h->adv_graph);
+  for (hipGraphExec_t &g : h->adv_graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   h->graph_exec = nullptr;
-  h->adv_graph = nullptr;
 }
 
-// one step with the dt the device chooses (LocalCFL, clamped to t_end)
-static void enqueue_adaptive_step(zfv_handle *h) {
+// one step with the dt the device chooses (LocalCFL, clamped to t_end);
+// with reject, the step is undone when the density changes too much
+static void enqueue_adaptive_step(zfv_handle *h, bool reject) {
   const Img &m = h->dimg;
+  const int64_t count = int64_t(h->img.s.nvars) * m.n;
   hipLaunchKernelGGL(k_adv_begin, dim3(1), dim3(1), 0, h->stream, h->adv);
+  if (reject)
+    hipLaunchKernelGGL(k_adv_backup, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->adv, count, h->U,
+                       h->Ubk);
   h->L.cfl(m, h->U, &h->adv->dt_bits, h->stream);
   hipLaunchKernelGGL(k_adv_choose, dim3(1), dim3(1), 0, h->stream, h->adv);
   const double *Us = h->U;
@@ -719,6 +776,13 @@ static void enqueue_adaptive_step(zfv_handle *h) {
     double *Un = (s == 2) ? h->U : h->X;
     h->L.update(m, ssp3_coeffs(s), s, 0.0, &h->adv->dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
     Us = h->X;
+  }
+  if (reject) {
+    // density is the first SoA plane of U
+    hipLaunchKernelGGL(k_adv_check, dim3(blocks_for(m.n, 256)), dim3(256), 0, h->stream, h->adv, m.n, h->U,
+                       h->Ubk);
+    hipLaunchKernelGGL(k_adv_restore, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->adv, count, h->U,
+                       h->Ubk);
   }
   hipLaunchKernelGGL(k_adv_end, dim3(1), dim3(1), 0, h->stream, h->adv);
 }
@@ -975,18 +1039,23 @@ int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
 }
 
 int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps, int32_t check_every,
-                double *t_final, int32_t *steps_done) {
+                double drho_crit_rel, int32_t max_exponent, double *t_final, int32_t *steps_done,
+                int32_t *rejected) {
   ZFV_API_BEGIN
   if (!(cfl_number > 0.0)) throw ZfvError(ZFV_ERR_INVALID, "cfl_number must be > 0");
   if (check_every < 1) check_every = 16;
+  if (drho_crit_rel > 0.0 && max_exponent < 0) throw ZfvError(ZFV_ERR_INVALID, "max_exponent must be >= 0");
   rec_config(h, h->U);
   if (!h->adv) h->adv = h->alloc<AdvState>(1);
-  if (!h->adv_graph) {
+  const bool reject = drho_crit_rel > 0.0;
+  if (reject && !h->Ubk) h->Ubk = h->alloc<double>(size_t(h->img.s.nvars) * h->img.n);
+  hipGraphExec_t &step = h->adv_graph[reject ? 1 : 0];
+  if (!step) {
     hipGraph_t graph;
     HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_adaptive_step(h);
+    enqueue_adaptive_step(h, reject);
     HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
-    HIP_CHECK(hipGraphInstantiate(&h->adv_graph, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphInstantiate(&step, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
   }
   AdvState a{};
@@ -996,15 +1065,20 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
   a.steps = 0;
   a.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
   a.active = t0 < t_end ? 1 : 0;
+  a.drho_crit = drho_crit_rel > 0.0 ? drho_crit_rel : 0.0;
+  a.factor = 1.0;
+  a.min_factor = drho_crit_rel > 0.0 ? std::pow(2.0, -double(max_exponent)) : 1.0;
+  a.prev_good = 1;
   HIP_CHECK(hipMemcpyAsync(h->adv, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   while (a.active) {
-    for (int k = 0; k < check_every; ++k) HIP_CHECK(hipGraphLaunch(h->adv_graph, h->stream));
+    for (int k = 0; k < check_every; ++k) HIP_CHECK(hipGraphLaunch(step, h->stream));
     HIP_CHECK(hipMemcpyAsync(&a, h->adv, sizeof(a), hipMemcpyDeviceToHost, h->stream));
     HIP_CHECK(hipStreamSynchronize(h->stream));
   }
   if (t_final) *t_final = a.t;
   if (steps_done) *steps_done = a.steps;
+  if (rejected) *rejected = a.rejections;
   return ZFV_OK;
   ZFV_API_END
 }
