@@ -165,6 +165,36 @@ def test_polytrope_3d_step_with_frozen_ghosts(gpu):
     assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize(
+    "name,make,params",
+    [
+        # 145,800 cells: more cells than one pass of the capped CFL grid
+        # covers, so the grid-stride loop and the block minima both count
+        # (order 2 keeps the oracle's setup short; LocalCFL ignores the order)
+        ("vortex2d_n270", lambda: W.vortex_2d(n=270), lambda wl: z.make_params(n_dims=2, order=2)),
+        ("polytrope3d_n12", lambda: W.polytrope_3d(n=12, amplitude=1e-3), lambda wl: wl.params),
+    ],
+)
+def test_cfl_dt_matches_oracle(gpu, name, make, params):
+    """LocalCFL (include/zisa/model/local_cfl_condition_impl.hpp:25-40):
+    cfl * min over all cells of inradius / (|v| + a)."""
+    wl = make()
+    p = params(wl)
+    s = z.Solver(wl.mesh, p)
+    o = Oracle(wl.mesh, p)
+    u = wl.u0.copy()
+    for state in ("u0", "one cell made fastest"):
+        s.set_state(u)
+        dg, do = s.cfl_dt(0.45), o.cfl_dt(u, 0.45)
+        print(name, state, "gpu", dg, "oracle", do)
+        assert abs(dg - do) <= 1e-14 * do
+        # the last cell gets the largest speed (internal energy kept): the
+        # minimum moves there
+        rho, m_old = u[-1, 0], u[-1, 1]
+        u[-1, 1] = 50.0 * rho * np.sqrt(p.gamma * 10.0)
+        u[-1, 4] += 0.5 * (u[-1, 1] ** 2 - m_old**2) / rho
+
+
 def test_launch_geometry_does_not_change_results(gpu):
     """The autotuner only picks cells per reconstruction block: every
     geometry gives bit-identical steps."""

@@ -152,10 +152,10 @@ __global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage
 template <int DIM>
 __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U, unsigned long long *out) {
   constexpr int NV = DIM + 2;
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = m.n;
   double r = 1.0e300;
-  if (i < m.n) {
-    const int64_t n = m.n;
+  // grid-stride over the cells: few blocks, so few atomics on one word
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     double u[NV];
     for (int v = 0; v < NV; ++v) u[v] = U[v * n + i];
     double m2 = u[1] * u[1] + u[2] * u[2];
@@ -163,8 +163,9 @@ __global__ __launch_bounds__(256) void k_cfl(Img m, const double *__restrict__ U
     double v2 = m2 / (u[0] * u[0]);
     double p = (u[NV - 1] - 0.5 * m2 / u[0]) * (m.gamma - 1.0);
     double a = sqrt(m.gamma * p / u[0]);
-    r = m.inr[i] / (sqrt(v2) + a);
-    if (!(r > 0.0)) r = 0.0;  // NaN / non-positive states surface as dt = 0
+    double ri = m.inr[i] / (sqrt(v2) + a);
+    if (!(ri > 0.0)) ri = 0.0;  // NaN / non-positive states surface as dt = 0
+    r = fmin(r, ri);
   }
   // wavefront reduction
   for (int off = 32; off > 0; off >>= 1) r = fmin(r, __shfl_down(r, off, 64));
@@ -388,12 +389,12 @@ struct AdvState {
   int bad, prev_good, retry, rejections;
 };
 
-__global__ void k_adv_begin(AdvState *a) {
+// start value of the CFL reduction (bits of 1e300)
+__host__ __device__ inline unsigned long long adv_no_bound() {
   const double big = 1.0e300;
   unsigned long long bits;
   memcpy(&bits, &big, sizeof(bits));
-  a->dt_bits = bits;
-  a->bad = 0;
+  return bits;
 }
 
 // keeps u0 for a step that may be rejected (only with step rejection on)
@@ -439,10 +440,14 @@ __global__ void k_adv_choose(AdvState *a) {
 
 __global__ void k_adv_end(AdvState *a) {
   if (!a->active) return;
+  const int bad = a->bad;
+  // the next step's CFL reduction and density check start here
+  a->dt_bits = adv_no_bound();
+  a->bad = 0;
   a->retry = 0;
   if (a->drho_crit > 0.0) {
     const double f = a->factor;
-    if (a->bad) {
+    if (bad) {
       a->prev_good = 0;
       a->factor = fmax(a->min_factor, fmin(f / 2.0, 1.0));
       a->retry = 1;
@@ -548,7 +553,7 @@ Launcher make_launcher() {
                        src, U0, K, Un);
   };
   l.cfl = [](const Img &m, const double *U, unsigned long long *out, hipStream_t s) {
-    hipLaunchKernelGGL((k_cfl<DIM>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, U, out);
+    hipLaunchKernelGGL((k_cfl<DIM>), dim3(std::min(blocks_for(m.n, 256), 512u)), dim3(256), 0, s, m, U, out);
   };
   l.brec[PH_LIGHT] = [](DevInst *d, const BlockRef *b, unsigned nb, int stage, int cfg, hipStream_t s) {
     launch_brec<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(d, b, nb, stage, cfg, s);
@@ -763,7 +768,6 @@ static void drop_graphs(zfv_handle *h) {
 static void enqueue_adaptive_step(zfv_handle *h, bool reject) {
   const Img &m = h->dimg;
   const int64_t count = int64_t(h->img.s.nvars) * m.n;
-  hipLaunchKernelGGL(k_adv_begin, dim3(1), dim3(1), 0, h->stream, h->adv);
   if (reject)
     hipLaunchKernelGGL(k_adv_backup, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->adv, count, h->U,
                        h->Ubk);
@@ -1069,6 +1073,8 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
   a.factor = 1.0;
   a.min_factor = drho_crit_rel > 0.0 ? std::pow(2.0, -double(max_exponent)) : 1.0;
   a.prev_good = 1;
+  a.dt_bits = adv_no_bound();
+  a.bad = 0;
   HIP_CHECK(hipMemcpyAsync(h->adv, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   while (a.active) {
