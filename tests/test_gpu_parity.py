@@ -79,6 +79,16 @@ def test_graph_steps_equal_eager_steps(gpu):
     b.steps(2, 0.0, dt)
     b.steps(2, 2 * dt, dt)
     np.testing.assert_array_equal(a.get_state(), b.get_state())
+    # more steps from the same graph
+    for k in range(19):
+        a.step((4 + k) * dt, dt)
+    b.steps(19, 4 * dt, dt)
+    np.testing.assert_array_equal(a.get_state(), b.get_state())
+    # a new dt recaptures the graph
+    for k in range(9):
+        a.step(0.0, 0.5 * dt)
+    b.steps(9, 0.0, 0.5 * dt)
+    np.testing.assert_array_equal(a.get_state(), b.get_state())
 
 
 def test_state_round_trip(gpu):
