@@ -69,3 +69,32 @@ def wb_err(a, b, mesh, u, gamma, mask=None):
         den = max(np.abs(b[mask, v]).max(), s)
         out.append(np.abs(a[mask, v] - b[mask, v]).max() / den)
     return out
+
+
+def shape_cases():
+    """One (mesh workload, params) pair per reconstruction kernel shape the
+    library compiles (stencil sizes x quadrature points): 2D orders 2, 3, 4
+    with both edge-rule sizes, 3D order 3 with both face rules and with the
+    reference's default of three one-sided stencils."""
+    import zisafvm_amd as z
+    from zisafvm_amd import workloads as W
+
+    def p3(**kw):
+        p = W.polytrope_3d(n=10, amplitude=1e-3).params
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    v2 = lambda: W.vortex_2d(n=14)
+    b3 = lambda: W.polytrope_3d(n=10, amplitude=1e-3)
+    return {
+        "2d_order3_edge_deg2": (v2, lambda: z.make_params(n_dims=2, order=3, edge_deg=2)),
+        "2d_order3_edge_deg4": (v2, lambda: z.make_params(n_dims=2, order=3, edge_deg=4)),
+        "2d_order4_edge_deg2": (v2, lambda: z.make_params(n_dims=2, order=4, edge_deg=2)),
+        "2d_order4_edge_deg4": (v2, lambda: z.make_params(n_dims=2, order=4, edge_deg=4)),
+        "2d_order2_edge_deg1": (v2, lambda: z.make_params(n_dims=2, order=2, edge_deg=1)),
+        "2d_order2_edge_deg2": (v2, lambda: z.make_params(n_dims=2, order=2, edge_deg=2)),
+        "3d_order3_face_deg2": (b3, lambda: p3()),
+        "3d_order3_face_deg3": (b3, lambda: p3(edge_deg=3)),
+        "3d_order3_three_one_sided": (b3, lambda: p3(n_stencils=4)),
+    }

@@ -92,3 +92,16 @@ def test_ghost_cells_evolve_like_the_reference_without_frozen_bc():
     ue = _ssp3(e.rate, u, dt, lambda v: v)
     errs = rel_err((ue - u) / dt, (o.step(u, 0.0, dt) - u) / dt)
     assert max(errs) <= TOL, errs
+
+
+@pytest.mark.parametrize("case", sorted(__import__("conftest").shape_cases()))
+def test_every_kernel_shape(case):
+    from conftest import shape_cases
+
+    make_wl, make_p = shape_cases()[case]
+    wl, p = make_wl(), make_p()
+    o, e = Oracle(wl.mesh, p), Emu(wl.mesh, p)
+    La, Lo = e.rate(wl.u0), o.rate(wl.u0)
+    # plain runs: relative to max |L_ref|; well-balanced runs: to the pressure scale
+    errs = rel_err(La, Lo) if p.gravity == 0 else wb_err(La, Lo, wl.mesh, wl.u0, p.gamma)
+    assert max(errs) <= TOL, errs

@@ -205,6 +205,22 @@ def test_cfl_dt_matches_oracle(gpu, name, make, params):
         u[-1, 4] += 0.5 * (u[-1, 1] ** 2 - m_old**2) / rho
 
 
+@pytest.mark.parametrize("case", sorted(__import__("conftest").shape_cases()))
+def test_every_kernel_shape_matches_oracle(gpu, case):
+    """Every compiled reconstruction shape against the oracle."""
+    from conftest import shape_cases, wb_err
+
+    make_wl, make_p = shape_cases()[case]
+    wl, p = make_wl(), make_p()
+    s = z.Solver(wl.mesh, p)
+    o = Oracle(wl.mesh, p)
+    La, Lo = s.rate(wl.u0), o.rate(wl.u0)
+    # plain runs: relative to max |L_ref|; well-balanced runs: to the pressure scale
+    errs = rel_err(La, Lo) if p.gravity == 0 else wb_err(La, Lo, wl.mesh, wl.u0, p.gamma)
+    print(case, "error per variable", errs)
+    assert max(errs) <= TOL
+
+
 def test_launch_geometry_does_not_change_results(gpu):
     """The autotuner only picks cells per reconstruction block: every
     geometry gives bit-identical steps."""
