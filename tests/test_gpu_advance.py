@@ -94,6 +94,7 @@ def _first_step_density_change(wl):
     [
         ("vortex2d", lambda: W.vortex_2d(n=32), 0.02),
         ("isothermal2d", lambda: W.isothermal_2d(n=24, amplitude=1e-3, order=3), 0.01),
+        ("polytrope3d", lambda: W.polytrope_3d(n=10, amplitude=1e-3), 0.05),
     ],
 )
 @pytest.mark.parametrize("check_every", [1, 16])

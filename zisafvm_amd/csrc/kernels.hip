@@ -713,25 +713,27 @@ static void tune_rec(zfv_handle *h, const double *Ustate) {
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
-  int best = kRecConfigs - 1;
-  double best_ms = 1e300;
-  for (int c = 0; c < kRecConfigs; ++c) {
-    h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
-    HIP_CHECK(hipEventRecord(e0, h->stream));
-    for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
-    HIP_CHECK(hipEventRecord(e1, h->stream));
-    HIP_CHECK(hipEventSynchronize(e1));
-    HIP_CHECK(hipGetLastError());
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    h->rec_ms[c] = ms / 5.0;
-    if (h->rec_ms[c] < best_ms) {
-      best_ms = h->rec_ms[c];
-      best = c;
+  // two rounds over the configurations, the faster round counting for each:
+  // the first launches after an idle spell run at lower clocks
+  for (int round = 0; round < 2; ++round)
+    for (int c = 0; c < kRecConfigs; ++c) {
+      h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
+      HIP_CHECK(hipEventRecord(e0, h->stream));
+      for (int k = 0; k < 5; ++k) h->L.rec[h->ph](m, Ustate, h->fpv, h->src, c, frozen(h), h->stream);
+      HIP_CHECK(hipEventRecord(e1, h->stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      HIP_CHECK(hipGetLastError());
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      h->rec_ms[c] = round == 0 ? ms / 5.0 : std::min(h->rec_ms[c], ms / 5.0);
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // the smallest block unless a larger one is clearly (2%) faster: near-ties
+  // are clock noise, and a fixed choice keeps runs comparable
+  int best = 0;
+  for (int c = 1; c < kRecConfigs; ++c)
+    if (h->rec_ms[c] < 0.98 * h->rec_ms[best]) best = c;
   h->rec_cfg = best;
 }
 
