@@ -222,8 +222,8 @@ def main():
                     help="run only the streaming-copy calibration kernel (1 GiB x 5) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over "
-              f"several GPUs is described in DESIGN.md, and no multi-GPU figure is produced.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): it measures one GPU's share of the "
+              f"work, and no multi-GPU figure is produced.", file=sys.stderr)
         return 2
 
     import zisafvm_amd as z

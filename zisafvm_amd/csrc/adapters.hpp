@@ -10,7 +10,8 @@
 // `AllVariables` here is a minimal stand-in with the reference's layout
 // (cvars n_cells x 5 row-major, src/zisa/model/all_variables.cpp:25-30); a
 // maintainer replaces it by zisa::AllVariables and the base classes by the
-// reference's own.  Header-only; link against
+// reference's own (include/zisa/ode/rate_of_change.hpp,
+// include/zisa/ode/time_integration.hpp).  Header-only; link against
 // libzfv_hip.so.
 #pragma once
 
