@@ -66,6 +66,35 @@ struct WaveGroup {
   }
 };
 
+// A group of 4 lanes (2D) is a DPP quad, lanes 4j .. 4j+3.  A broadcast whose
+// lane index the compiler knows (unrolled loops) is a quad_perm move in the
+// VALU, with no LDS round trip; a run-time index keeps the LDS permute (a
+// select over the four moves measured slower).  The values are the same.
+struct QuadGroup : WaveGroup {
+  template <int K>
+  __device__ __forceinline__ static int q32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
+  }
+  __device__ __forceinline__ static int sel32(int v, int k) {
+    switch (k) {
+      case 0: return q32<0>(v);
+      case 1: return q32<1>(v);
+      case 2: return q32<2>(v);
+      default: return q32<3>(v);
+    }
+  }
+  __device__ __forceinline__ double bcast(double v, int k) const {
+    if (!__builtin_constant_p(k)) return __shfl(v, base + k, 64);
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(sel32(int(b), k)), hi = unsigned(sel32(int(b >> 32), k));
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+  }
+  __device__ __forceinline__ int32_t bcast_i(int32_t v, int k) const {
+    if (!__builtin_constant_p(k)) return __shfl(v, base + k, 64);
+    return sel32(v, k);
+  }
+};
+
 // groups of NV lanes per 64-wide wavefront (2D: 16 cells, 3D: 12 cells)
 template <int NV>
 constexpr int groups_per_wave() {
@@ -91,8 +120,14 @@ __global__ __launch_bounds__(256) void k_reconstruct(
   // under the frozen BC a ghost cell away from the interior feeds no flux and
   // its own tendency is overwritten: nothing to reconstruct
   if (skip_far_ghosts && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()]};
-  reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
+  double *sl = &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()];
+  if constexpr (NV == 4) {
+    QuadGroup x{{lane - g, sl}};
+    reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
+  } else {
+    WaveGroup x{lane - g, sl};
+    reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
+  }
 }
 
 template <int DIM, int Q>
