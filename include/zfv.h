@@ -110,6 +110,17 @@ int zfv_destroy(zfv_handle *h);
 int zfv_set_state(zfv_handle *h, const double *cvars);
 int zfv_get_state(zfv_handle *h, double *cvars);
 
+/* Snapshot for output while the run goes on (the state TimeLoop::write_output
+ * hands to its visualization, src/zisa/fvm_loops/time_loop.cpp:83-85,163-167;
+ * AllVariables::cvars, src/zisa/model/all_variables.cpp:74-90).  begin: the
+ * state as of all work issued on the handle so far is copied on the device,
+ * then to pinned host memory on a second stream, while later steps proceed.
+ * end: waits for that copy and writes it like zfv_get_state.  One snapshot
+ * at a time (begin while one is pending is an error).  The first begin
+ * allocates a device and a pinned host copy of the state. */
+int zfv_snapshot_begin(zfv_handle *h);
+int zfv_snapshot_end(zfv_handle *h, double *cvars);
+
 /* FrozenBC steady state (src/zisa/boundary/frozen_boundary_condition.cpp:11-37). */
 int zfv_set_steady_state(zfv_handle *h, const double *cvars);
 

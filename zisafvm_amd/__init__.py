@@ -204,6 +204,8 @@ def load_library(path: str = LIB_PATH):
         "zfv_advance": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, c_double,
                                 ctypes.c_int32, dp, ip, ip]),
         "zfv_synchronize": (c_int, [vp]),
+        "zfv_snapshot_begin": (c_int, [vp]),
+        "zfv_snapshot_end": (c_int, [vp, dp]),
         "zfv_get_info": (c_int, [vp, ctypes.POINTER(Info)]),
         "zfv_get_stencils": (c_int, [vp, ctypes.c_int32, ip, ctypes.c_int32, ip, ip, ctypes.c_int32]),
         "zfv_time_kernels": (c_int, [vp, ctypes.c_int32, c_double, ctypes.POINTER(KernelTimes)]),
@@ -356,6 +358,17 @@ class Solver:
     def get_state(self) -> np.ndarray:
         u = np.zeros((self.n_cells, 5))
         _check(self.lib.zfv_get_state(self.h, _dp(u)))
+        return u
+
+    def snapshot_begin(self):
+        """Starts copying the current device state off the device while later
+        steps proceed (zfv_snapshot_begin)."""
+        _check(self.lib.zfv_snapshot_begin(self.h))
+
+    def snapshot_end(self) -> np.ndarray:
+        """The state at the last snapshot_begin (zfv_snapshot_end)."""
+        u = np.zeros((self.n_cells, 5))
+        _check(self.lib.zfv_snapshot_end(self.h, _dp(u)))
         return u
 
     def set_steady_state(self, u: np.ndarray):

@@ -660,6 +660,11 @@ struct zfv_handle {
   int rec_cfg = -1;   // reconstruction launch configuration; -1 = not tuned yet
   double rec_ms[zfv::kRecConfigs] = {0, 0, 0};
   int ph = 1;         // physics set of the reconstruction kernel (PH_LIGHT / PH_FULL / PH_LIN)
+  // asynchronous snapshot of the state (zfv_snapshot_begin / zfv_snapshot_end)
+  double *snap_dev = nullptr, *snap_host = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ready = nullptr;
+  bool snap_pending = false;
 
   template <class T>
   T *alloc(size_t count) {
@@ -677,11 +682,16 @@ struct zfv_handle {
     return p;
   }
   ~zfv_handle() {
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (hipGraphExec_t g : adv_graph)
       if (g) (void)hipGraphExecDestroy(g);
     for (void *p : allocations) (void)hipFree(p);
     if (host_stage) (void)hipHostFree(host_stage);
+    if (snap_host) (void)hipHostFree(snap_host);
+    if (snap_ready) (void)hipEventDestroy(snap_ready);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -1005,6 +1015,38 @@ int zfv_get_state(zfv_handle *h, double *cvars) {
   HIP_CHECK(hipMemcpyAsync(h->host_stage, h->U, bytes, hipMemcpyDeviceToHost, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   from_soa(h, h->host_stage, cvars, false);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_snapshot_begin(zfv_handle *h) {
+  ZFV_API_BEGIN
+  if (h->snap_pending) throw ZfvError(ZFV_ERR_INVALID, "a snapshot is pending: call zfv_snapshot_end first");
+  const size_t count = size_t(h->img.s.nvars) * h->img.n;
+  if (!h->snap_dev) {
+    h->snap_dev = h->alloc<double>(count);
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->snap_host), sizeof(double) * count));
+    HIP_CHECK(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&h->snap_ready, hipEventDisableTiming));
+  }
+  // the state as of the work enqueued so far, copied on the device in stream
+  // order; the copy to the host runs on its own stream behind later steps
+  HIP_CHECK(hipMemcpyAsync(h->snap_dev, h->U, sizeof(double) * count, hipMemcpyDeviceToDevice, h->stream));
+  HIP_CHECK(hipEventRecord(h->snap_ready, h->stream));
+  HIP_CHECK(hipStreamWaitEvent(h->copy_stream, h->snap_ready, 0));
+  HIP_CHECK(hipMemcpyAsync(h->snap_host, h->snap_dev, sizeof(double) * count, hipMemcpyDeviceToHost,
+                           h->copy_stream));
+  h->snap_pending = true;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_snapshot_end(zfv_handle *h, double *cvars) {
+  ZFV_API_BEGIN
+  if (!h->snap_pending) throw ZfvError(ZFV_ERR_INVALID, "no snapshot pending");
+  h->snap_pending = false;
+  HIP_CHECK(hipStreamSynchronize(h->copy_stream));
+  from_soa(h, h->snap_host, cvars, false);
   return ZFV_OK;
   ZFV_API_END
 }
