@@ -252,6 +252,17 @@ def main():
     t0 = time.perf_counter()
     _, adv_steps = s.advance(wl.cfl, 0.0, 1.0e9, max_steps=args.steps, check_every=args.steps)
     el_adv = time.perf_counter() - t0
+    # the same fixed-dt steps with a snapshot of the state draining to the
+    # host on a second stream behind them (zfv_snapshot_begin/_end)
+    s.snapshot_begin()
+    s.snapshot_end()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.snapshot_begin()
+    s.steps(args.steps, 0.0, dt)
+    t1 = time.perf_counter()
+    s.snapshot_end()
+    el_snap, snap_end_ms = t1 - t0, 1e3 * (time.perf_counter() - t1)
     # the drop-in RateOfChange::compute path: host state in, host tendency out
     # (AoS <-> SoA on the host and PCIe both ways); reported beside, never value
     dudt = wl.u0 * 0.0
@@ -315,6 +326,10 @@ def main():
         "device_time_loop": {"steps": adv_steps, "ms_per_step": 1e3 * el_adv / max(adv_steps, 1),
                              "cell_stage_per_s": n * 3 * adv_steps / el_adv,
                              "note": "zfv_advance: CFL dt chosen on the device every step"},
+        "snapshot_overlap": {"ms_per_step": 1e3 * el_snap / args.steps, "state_mb": n * 5 * 8 / 1e6,
+                             "end_ms": snap_end_ms,
+                             "note": "the fixed-dt steps with a snapshot copied to the host behind them; "
+                                     "end_ms: zfv_snapshot_end after them (wait + reorder on the host)"},
         "host_buffer_rate": {"ms_per_call": rate_ms, "cell_stage_per_s": n / (rate_ms * 1e-3),
                              "note": "zfv_rate on host buffers (one RoC evaluation incl. PCIe and layout "
                                      "conversion), the drop-in RateOfChange::compute path"},
