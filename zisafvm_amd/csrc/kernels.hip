@@ -838,28 +838,10 @@ static void enqueue_adaptive_step(zfv_handle *h, bool reject) {
   hipLaunchKernelGGL(k_adv_end, dim3(1), dim3(1), 0, h->stream, h->adv);
 }
 
-static void to_soa(const zfv_handle *h, const double *aos, double *soa) {
-  const int64_t n = h->img.n;
-  const int nv = h->img.s.nvars;
-  const int dim = h->img.s.dim;
-  for (int64_t ip = 0; ip < n; ++ip) {
-    const double *u = aos + 5 * int64_t(h->img.perm[ip]);
-    soa[0 * n + ip] = u[0];
-    for (int d = 0; d < dim; ++d) soa[(1 + d) * n + ip] = u[1 + d];
-    soa[(nv - 1) * n + ip] = u[4];
-  }
-}
+static void to_soa(const zfv_handle *h, const double *aos, double *soa) { zfv::to_soa(h->img, aos, soa); }
 
 static void from_soa(const zfv_handle *h, const double *soa, double *aos, bool accumulate) {
-  const int64_t n = h->img.n;
-  const int nv = h->img.s.nvars;
-  const int dim = h->img.s.dim;
-  for (int64_t ip = 0; ip < n; ++ip) {
-    double *u = aos + 5 * int64_t(h->img.perm[ip]);
-    double w[5] = {soa[0 * n + ip], 0.0, 0.0, 0.0, soa[(nv - 1) * n + ip]};
-    for (int d = 0; d < dim; ++d) w[1 + d] = soa[(1 + d) * n + ip];
-    for (int v = 0; v < 5; ++v) u[v] = accumulate ? u[v] + w[v] : w[v];
-  }
+  zfv::from_soa(h->img, soa, aos, accumulate);
 }
 
 // launch configuration named by its threads per block

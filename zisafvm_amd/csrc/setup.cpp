@@ -1103,4 +1103,29 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   for (int k = 0; k < p.n_stencils; ++k) img.linear_weights[k] = p.linear_weights[k] / tot;
 }
 
+void to_soa(const HostImage &img, const double *aos, double *soa) {
+  const int64_t n = img.n;
+  const int nv = img.s.nvars, dim = img.s.dim;
+#pragma omp parallel for schedule(static)
+  for (int64_t ip = 0; ip < n; ++ip) {
+    const double *u = aos + 5 * int64_t(img.perm[ip]);
+    soa[ip] = u[0];
+    for (int d = 0; d < dim; ++d) soa[(1 + d) * n + ip] = u[1 + d];
+    soa[(nv - 1) * n + ip] = u[4];
+  }
+}
+
+void from_soa(const HostImage &img, const double *soa, double *aos, bool accumulate) {
+  const int64_t n = img.n;
+  const int nv = img.s.nvars, dim = img.s.dim;
+  // perm is a permutation: every iteration writes its own row
+#pragma omp parallel for schedule(static)
+  for (int64_t ip = 0; ip < n; ++ip) {
+    double *u = aos + 5 * int64_t(img.perm[ip]);
+    double w[5] = {soa[ip], 0.0, 0.0, 0.0, soa[(nv - 1) * n + ip]};
+    for (int d = 0; d < dim; ++d) w[1 + d] = soa[(1 + d) * n + ip];
+    for (int v = 0; v < 5; ++v) u[v] = accumulate ? u[v] + w[v] : w[v];
+  }
+}
+
 }  // namespace zfv

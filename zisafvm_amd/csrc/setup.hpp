@@ -81,4 +81,10 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img);
 // instantiation for it).
 Shape shape_for(const zfv_params &p);
 
+// The caller's n x 5 row-major state (AllVariables::cvars) to and from the
+// product's nvars x n planes in product order (perm: product index -> caller
+// index), on the host's threads.  from_soa adds into aos when accumulate.
+void to_soa(const HostImage &img, const double *aos, double *soa);
+void from_soa(const HostImage &img, const double *soa, double *aos, bool accumulate);
+
 }  // namespace zfv
