@@ -117,7 +117,8 @@ int zfv_get_state(zfv_handle *h, double *cvars);
  * then to pinned host memory on a second stream, while later steps proceed.
  * end: waits for that copy and writes it like zfv_get_state.  One snapshot
  * at a time (begin while one is pending is an error).  The first begin
- * allocates a device and a pinned host copy of the state. */
+ * allocates a device and a pinned host copy of the state.  A batch member's
+ * state is complete whenever zfv_batch_run has returned. */
 int zfv_snapshot_begin(zfv_handle *h);
 int zfv_snapshot_end(zfv_handle *h, double *cvars);
 
