@@ -42,6 +42,9 @@ enum { ZFV_CWENO_AO = 0, ZFV_WENO_AO = 1 };
 enum { ZFV_FLUX_HLLC = 0, ZFV_FLUX_RUSANOV = 1 };
 enum { ZFV_EQ_NONE = 0, ZFV_EQ_ISENTROPIC = 1, ZFV_EQ_ISOTHERMAL = 2 };
 enum { ZFV_GRAVITY_NONE = 0, ZFV_GRAVITY_AXIAL = 1, ZFV_GRAVITY_RADIAL = 2, ZFV_GRAVITY_POLYTROPE = 3 };
+/* flux-bc.mode "none" / "constant" (FluxBC) / "isentropic" (EquilibriumFluxBC)
+ * (include/zisa/experiments/euler_experiment_impl.hpp:277-299) */
+enum { ZFV_FLUX_BC_NONE = 0, ZFV_FLUX_BC_CONSTANT = 1, ZFV_FLUX_BC_ISENTROPIC = 2 };
 
 enum {
   ZFV_OK = 0,
@@ -77,7 +80,10 @@ typedef struct zfv_params {
   double gamma;                   /* ideal gas */
   double gas_constant;
   int32_t edge_deg, volume_deg, moments_deg;  /* quadrature degrees, -1 = max */
-  int32_t flux_bc;                /* 0 none, 1 constant-extrapolation FluxBC */
+  int32_t flux_bc;                /* ZFV_FLUX_BC_*: 0 none, 1 FluxBC (the cell average's
+                                     physical flux), 2 EquilibriumFluxBC (the flux of the
+                                     cell's isentropic equilibrium at the face points;
+                                     with ZFV_EQ_ISENTROPIC only) */
   int32_t steps_per_recompute;    /* must be 1: equilibria are re-solved every call */
   int32_t scaling;                /* 0 EulerScaling (the experiments' choice), 1 UnityScaling
                                      (include/zisa/model/characteristic_scale.hpp:14-46);

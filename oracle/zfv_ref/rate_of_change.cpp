@@ -735,6 +735,40 @@ void Solver::rate(const double *u, double *dudt, double /*t*/) {
       for (int v = 0; v < 5; ++v) dudt[5 * size_t(i) + v] -= fac * f[v];
     }
   }
+
+  // EquilibriumFluxBC<IsentropicEquilibrium> (include/zisa/boundary/equilibrium_flux_bc.hpp:36-62):
+  // per exterior face, the isentropic equilibrium of the owner cell solved
+  // from its average (LocalEquilibrium::solve -> solve_exact), evaluated at
+  // the face points as a state at rest, whose Euler flux in the face frame is
+  // (0, p, 0, 0, 0); rotated back, integrated with the face rule and
+  // subtracted with 1/vol.  An equilibrium that is not found extrapolates to
+  // zeros, whose pressure is NaN (0/0), as in the reference.
+  if (params_.flux_bc == 2) {
+    for (int e = g.n_interior_edges; e < g.n_edges; ++e) {
+      int i = g.left_right[e].first;
+      const Face &face = g.faces[e];
+      const Vars ui = load(u, i);
+      LocalEquilibrium eq;
+      eq.kind = 1;
+      eq.eos = &im.eos;
+      eq.gravity = &im.gravity;
+      eq.solve(RhoE{ui[0], internal_energy(ui)}, g.cells[i]);
+      auto flux = [&](const XYZ &x) {
+        RhoE r = eq.extrapolate(x);
+        Vars uu{r.rho, 0.0, 0.0, 0.0, r.E};
+        Vars f = euler_flux(uu, pressure(im.eos, uu));
+        inv_coord_transform(f, face);
+        return f;
+      };
+      Vars f = flux(face.qr.points[0]);
+      for (int v = 0; v < 5; ++v) f[v] = face.qr.weights[0] * f[v];
+      for (size_t q = 1; q < face.qr.weights.size(); ++q) {
+        Vars t = flux(face.qr.points[q]);
+        for (int v = 0; v < 5; ++v) f[v] = f[v] + face.qr.weights[q] * t[v];
+      }
+      for (int v = 0; v < 5; ++v) dudt[5 * size_t(i) + v] -= f[v] / g.volumes[i];
+    }
+  }
 }
 
 // RungeKutta::compute_step + runge_kutta_sum + FrozenBC::apply

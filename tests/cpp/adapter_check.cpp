@@ -79,6 +79,26 @@ int main() {
     return 1;
   }
 
+  // TimeLoop may change the returned state in place before the next step
+  // (instantaneous physics, src/zisa/fvm_loops/time_loop.cpp:60): the next
+  // step must start from the changed values, not from the device copy
+  auto inst3 = std::make_shared<Instance>(mesh, p);
+  HipRungeKutta rk3(inst3, n);
+  auto v0 = std::make_shared<AllVariables>(before);
+  auto v1 = rk3.compute_step(v0, t, dt);
+  for (int i = 0; i < n; ++i) v1->cvars[5 * size_t(i) + 4] *= 1.001;  // heat every cell
+  const AllVariables changed = *v1;
+  auto v2 = rk3.compute_step(v1, t + dt, dt);
+  auto inst4 = std::make_shared<Instance>(mesh, p);
+  check(zfv_set_state(inst4->get(), changed.data()));
+  check(zfv_step(inst4->get(), t + dt, dt));
+  AllVariables ref4(n);
+  check(zfv_get_state(inst4->get(), ref4.data()));
+  if (ref4.cvars != v2->cvars) {
+    std::printf("adapter_check: a step after an in-place change of the state did not start from it\n");
+    return 1;
+  }
+
   // the same two steps through the C-ABI on a fresh instance
   auto inst2 = std::make_shared<Instance>(mesh, p);
   check(zfv_set_state(inst2->get(), before.data()));

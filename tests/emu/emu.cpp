@@ -281,6 +281,65 @@ int zfv_emu_stencils(void *h, int i, int *l2g, int cap, int *l2g_size, int *memb
   return mb - ma;
 }
 
+// Every index the kernels derive from the instance's setup data, checked
+// against the array it addresses: stencil row entries (cell < n, translation
+// code inside the table), twin rows, face words (face id < ne, ghost pairs),
+// face-vertex permutations, the CWENO index.  Returns 0 when all hold, else
+// writes the first violation into msg.
+int zfv_emu_validate(void *h, char *msg, int cap) {
+  const Emu &e = *static_cast<Emu *>(h);
+  const HostImage &im = e.img;
+  const Shape &s = im.s;
+  const int64_t n = im.n, ne = im.ne;
+  std::string err;
+  auto bad = [&](const std::string &what, int64_t i) {
+    if (err.empty()) err = what + " at cell " + std::to_string(i);
+  };
+  int n_codes = 1;
+  for (int q = 1; q < 16; ++q)
+    if (im.shift[3 * q] != 0.0 || im.shift[3 * q + 1] != 0.0 || im.shift[3 * q + 2] != 0.0) n_codes = q + 1;
+  if (int64_t(im.rows.size()) != int64_t(s.nrows) * n) bad("rows size", -1);
+  if (int64_t(im.cface.size()) != int64_t(s.nfaces) * n) bad("cface size", -1);
+  std::vector<int> face_uses(size_t(ne > 0 ? ne : 1), 0);
+  for (int64_t i = 0; i < n && err.empty(); ++i) {
+    for (int r = 0; r < s.nrows; ++r) {
+      const int32_t rw = im.rows[size_t(r) * n + i];
+      if (rw < 0) bad("negative row entry", i);
+      const int64_t j = rw & ROW_CELL_MASK;
+      const int code = int((uint32_t(rw) >> ROW_CELL_BITS) & 15u);
+      if (j >= n) bad("row cell out of range", i);
+      if (code >= n_codes) bad("row translation code outside the table", i);
+    }
+    for (int r = s.rc; r < s.nrows; ++r) {
+      const int t = im.twin[size_t(r - s.rc) * n + i];
+      if (t < -1 || t >= s.rc) bad("twin row out of range", i);
+      if (t >= 0 && im.rows[size_t(t) * n + i] != im.rows[size_t(r) * n + i]) bad("twin row names another member", i);
+    }
+    const int32_t meta = im.meta[i];
+    if ((meta & 7) > s.nb) bad("CWENO stencil index out of range", i);
+    for (int f = 0; f < s.nfaces; ++f) {
+      const int32_t cf = im.cface[size_t(f) * n + i];
+      if ((cf & 2) == 0) {
+        const int64_t fe = cf >> 2;
+        if (fe < 0 || fe >= ne) bad("face id out of range", i);
+        else face_uses[size_t(fe)] += 1;
+      } else if (cf != 2 && cf != 3) {
+        bad("exterior face word with bits above the flags", i);
+      }
+      for (int r = 0; r < s.dim; ++r)
+        if (int((im.fperm[i] >> (2 * (f * 3 + r))) & 3u) > s.dim) bad("face vertex index out of range", i);
+    }
+  }
+  for (int64_t fe = 0; fe < ne && err.empty(); ++fe)
+    if (face_uses[size_t(fe)] != 2) bad("interior face " + std::to_string(fe) + " is not shared by exactly two cells", -1);
+  if (err.empty()) return 0;
+  if (msg && cap > 0) {
+    std::strncpy(msg, err.c_str(), size_t(cap) - 1);
+    msg[cap - 1] = 0;
+  }
+  return 1;
+}
+
 int zfv_emu_info(void *h, int64_t *n, int64_t *ne, double *max_cond) {
   const HostImage &im = static_cast<Emu *>(h)->img;
   *n = im.n;

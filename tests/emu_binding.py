@@ -28,6 +28,7 @@ def load():
     lib.zfv_emu_rate.argtypes = [vp, dp, dp]
     lib.zfv_emu_stencils.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int, ip, ip, ctypes.c_int]
     lib.zfv_emu_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), dp]
+    lib.zfv_emu_validate.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     lib.zfv_emu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -62,6 +63,12 @@ class Emu:
         w = self.lib.zfv_emu_stencils(self.h, i, l2g, 512, ctypes.byref(size), mem, 1024)
         assert w >= 0
         return list(l2g[: size.value]), list(mem[:w])
+
+    def validate(self):
+        """None when every index the kernels derive from the setup data is in
+        range, else the first violation."""
+        buf = ctypes.create_string_buffer(256)
+        return None if self.lib.zfv_emu_validate(self.h, buf, 256) == 0 else buf.value.decode()
 
     def info(self):
         n, ne, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()

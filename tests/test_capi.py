@@ -104,3 +104,39 @@ def test_mesh_generation_is_deterministic():
     np.testing.assert_array_equal(a.corners, b.corners)
     c = z.lattice_mesh_2d(8, seed=43)
     assert not np.array_equal(a.corners, c.corners)
+
+
+def _estimate(mesh, p):
+    from zisafvm_amd import serving as S
+
+    return S.estimate_device_bytes(mesh, p)
+
+
+def test_enumerated_fields_are_range_checked():
+    # an unknown mode is an error (the reference's factories LOG_ERR on
+    # unknown modes), never another physics; shape_for runs before any device
+    # work, so zfv_estimate_device_bytes reports it on the CPU
+    mesh = z.lattice_mesh_2d(6)
+    assert _estimate(mesh, z.make_params()) > 0
+    bad = [("flux_bc", 3), ("flux_bc", -1), ("flux", 2), ("reconstruction", 2), ("equilibrium", 3), ("gravity", 4),
+           ("gamma", 1.0)]
+    for field, value in bad:
+        p = z.make_params()
+        setattr(p, field, value)
+        try:
+            _estimate(mesh, p)
+        except z.ZfvError as e:
+            assert field.split("_")[0] in str(e), (field, str(e))
+        else:
+            raise AssertionError(f"accepted {field} = {value}")
+    # the isentropic flux BC rides on the isentropic well-balancing
+    p = z.make_params(flux_bc=z.ZFV_FLUX_BC_ISENTROPIC)
+    try:
+        _estimate(mesh, p)
+    except z.ZfvError as e:
+        assert "isentropic" in str(e)
+    else:
+        raise AssertionError("accepted the isentropic flux BC without isentropic well-balancing")
+    p = z.make_params(flux_bc=z.ZFV_FLUX_BC_ISENTROPIC, equilibrium=z.ZFV_EQ_ISENTROPIC, gravity=z.ZFV_GRAVITY_AXIAL,
+                      gravity_g=1.0)
+    assert _estimate(mesh, p) > 0

@@ -105,3 +105,36 @@ def test_every_kernel_shape(case):
     # plain runs: relative to max |L_ref|; well-balanced runs: to the pressure scale
     errs = rel_err(La, Lo) if p.gravity == 0 else wb_err(La, Lo, wl.mesh, wl.u0, p.gamma)
     assert max(errs) <= TOL, errs
+
+
+@pytest.mark.parametrize("flux_bc", [0, 1, 2])
+@pytest.mark.parametrize("amp", [1e-3, 0.0])
+def test_isentropic_general_gamma(flux_bc, amp):
+    # the reference's isentropic equilibrium at gamma 1.4 (power laws at every
+    # point, the full physics set) on a mesh periodic in x (translated stencil
+    # members) with solid walls closed by each flux BC
+    wl = W.isentropic_2d(n=12, amplitude=amp, flux_bc=flux_bc)
+    o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)
+    errs = wb_err(e.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, wl.params.gamma)
+    assert max(errs) <= TOL, errs
+
+
+def test_gravity_without_equilibrium():
+    # GravitySourceLoop's NoEquilibrium form: average(cell, (0, -rho grad phi,
+    # -m . grad phi)) (include/zisa/fvm_loops/gravity_source_loop.hpp:121-148)
+    mesh = z.lattice_mesh_2d(12, jitter=0.2, seed=4, periodic=(True, False), ghost_layers=3)
+    u = W.cell_average(mesh, lambda x: W.isentropic_vortex(x, beta=3.0))
+    p = z.make_params(n_dims=2, order=3, gravity=z.ZFV_GRAVITY_AXIAL, gravity_g=1.0, gravity_axis=(0.0, 1.0, 0.0),
+                      flux_bc=1)
+    o, e = Oracle(mesh, p), Emu(mesh, p)
+    errs = rel_err(e.rate(u), o.rate(u), mask=mesh.ghost == 0)
+    assert max(errs) <= TOL, errs
+
+
+def test_isentropic_gamma2_periodic():
+    # gamma 2 with periodic images: the full set, with moment averages for
+    # untranslated members and point evaluations for translated ones
+    wl = W.isentropic_2d(n=12, amplitude=1e-4, gamma=2.0, flux_bc=2)
+    o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)
+    errs = wb_err(e.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, wl.params.gamma)
+    assert max(errs) <= TOL, errs

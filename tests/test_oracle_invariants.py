@@ -84,6 +84,46 @@ def test_uniform_state_has_zero_rate():
     assert np.abs(L).max() < 1e-11
 
 
+@pytest.mark.parametrize("flux_bc", [0, 1])
+def test_uniform_state_with_boundaries(flux_bc):
+    # FluxBC (include/zisa/boundary/flux_bc.hpp:25-42): with the physical
+    # flux of the cell average at every boundary face, a uniform state on a
+    # mesh with boundaries has a zero rate (each closed cell's face normals
+    # sum to zero); without it (flux_bc 0) the boundary cells lose exactly
+    # the missing face's flux.  A wrong sign doubles that flux instead.
+    mesh = z.lattice_mesh_2d(10, jitter=0.2, seed=8, periodic=(False, False), ghost_layers=0)
+    u = np.tile([1.2, 0.3, -0.4, 0.0, 3.0], (mesh.n_cells, 1))
+    p = z.make_params(n_dims=2, order=3, flux_bc=flux_bc)
+    L = Oracle(mesh, p).rate(u)
+    if flux_bc == 1:
+        assert np.abs(L).max() < 1e-11
+    else:
+        # the boundary cells lose the flux through their boundary faces; the
+        # others still see a closed, uniform neighbourhood
+        assert np.abs(L).max() > 0.1
+        interior_rows = np.abs(L).max(axis=1) < 1e-11
+        assert interior_rows.sum() > 0.5 * mesh.n_cells
+
+
+def test_equilibrium_flux_bc_keeps_isentropic_atmosphere_at_rest():
+    # EquilibriumFluxBC (include/zisa/boundary/equilibrium_flux_bc.hpp:36-62)
+    # closes the well-balanced scheme at solid boundaries: at a discrete
+    # isentropic equilibrium (gamma 1.4, walls at y = 0 and y = 1) the flux of
+    # each boundary cell's own equilibrium balances the pressure term of the
+    # well-balanced source at that face, so every cell, boundary rows included,
+    # has a round-off rate.  The constant FluxBC and no BC leave the boundary
+    # rows O(1) out of balance.
+    out = {}
+    for fbc in (0, 1, 2):
+        wl = W.isentropic_2d(n=12, flux_bc=fbc)
+        o = Oracle(wl.mesh, wl.params)
+        out[fbc] = (o.rate(wl.u0), _wb_flux_scale(wl, o))
+    L2, scale = out[2]
+    assert np.abs(L2).max() <= 1e-13 * scale
+    for fbc in (0, 1):
+        assert np.abs(out[fbc][0]).max() > 1e-3 * scale
+
+
 def test_conservation_on_periodic_mesh():
     # interior fluxes telescope: sum_i vol_i L_i = 0 up to round-off
     wl = W.vortex_2d(n=16)

@@ -28,6 +28,7 @@ ZFV_CWENO_AO, ZFV_WENO_AO = 0, 1
 ZFV_FLUX_HLLC, ZFV_FLUX_RUSANOV = 0, 1
 ZFV_EQ_NONE, ZFV_EQ_ISENTROPIC, ZFV_EQ_ISOTHERMAL = 0, 1, 2
 ZFV_GRAVITY_NONE, ZFV_GRAVITY_AXIAL, ZFV_GRAVITY_RADIAL, ZFV_GRAVITY_POLYTROPE = 0, 1, 2, 3
+ZFV_FLUX_BC_NONE, ZFV_FLUX_BC_CONSTANT, ZFV_FLUX_BC_ISENTROPIC = 0, 1, 2
 
 
 class Params(ctypes.Structure):

@@ -15,6 +15,7 @@
 // libzfv_hip.so.
 #pragma once
 
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -79,8 +80,11 @@ class TimeIntegration {
 // device.  Keeps the reference's buffer contract: u0 is not modified, the
 // returned buffer is the previous scratch buffer and u0 becomes the next
 // scratch (src/zisa/ode/runge_kutta.cpp:87-112).  The device holds the
-// state between calls; it is uploaded only when the caller passes a buffer
-// other than the one returned last.
+// state between calls.  The caller may change the returned buffer in place
+// before passing it back (TimeLoop applies its instantaneous physics to it,
+// src/zisa/fvm_loops/time_loop.cpp:60), so the upload is skipped only when
+// u0 is the buffer returned last and still holds, bit for bit, what was
+// returned; anything else is uploaded.
 class HipRungeKutta : public TimeIntegration {
  public:
   explicit HipRungeKutta(std::shared_ptr<Instance> inst, int n_cells)
@@ -88,9 +92,12 @@ class HipRungeKutta : public TimeIntegration {
 
   std::shared_ptr<AllVariables> compute_step(const std::shared_ptr<AllVariables> &u0, double t,
                                              double dt) override {
-    if (u0.get() != last_.get()) check(zfv_set_state(inst_->get(), u0->data()));
+    const bool resident = u0.get() == last_.get() && u0->cvars.size() == returned_.size() &&
+                          std::memcmp(u0->data(), returned_.data(), returned_.size() * sizeof(double)) == 0;
+    if (!resident) check(zfv_set_state(inst_->get(), u0->data()));
     check(zfv_step(inst_->get(), t, dt));
     check(zfv_get_state(inst_->get(), ux_->data()));
+    returned_ = ux_->cvars;
     auto out = ux_;
     ux_ = u0;
     last_ = out;
@@ -100,6 +107,7 @@ class HipRungeKutta : public TimeIntegration {
  private:
   std::shared_ptr<Instance> inst_;
   std::shared_ptr<AllVariables> ux_, last_;
+  std::vector<double> returned_;  // the values handed out last (the device state)
 };
 
 // LocalCFL on the device-resident state.

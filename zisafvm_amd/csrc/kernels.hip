@@ -145,9 +145,12 @@ struct StageCoeffs {
 
 // k_update: K[stage] = L(Ustage); Unext = U0 + dt * sum_j c_j K[j]
 // (src/zisa/ode/runge_kutta.cpp:122-143), then FrozenBC for ghost cells.
+// Ustage, U0 and Unext may name the same buffer (stages 1 and 2 read and
+// write the stage state X; a thread reads only its own cell's entries before
+// it writes them), so none of them is declared __restrict__.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage, double dt, const double *dtp,
-                                                const double *__restrict__ Ustage, const double *__restrict__ G,
+                                                const double *Ustage, const double *__restrict__ G,
                                                 const double *__restrict__ src, const double *U0,
                                                 double *__restrict__ K, double *Unext) {
   constexpr int NV = DIM + 2;

@@ -841,6 +841,22 @@ Shape shape_for(const zfv_params &p) {
   s.nmom = std::max(0, s.dofc - 3);
   s.q = int((s.dim == 2 ? gl_edge(p.edge_deg) : dunavant(p.edge_deg)).w.size());
   s.qv = int(volume_rule(s.dim, p.volume_deg).w.size());
+  // the reference rejects unknown modes (LOG_ERR in its factories); so does
+  // every enumerated field here, rather than running another physics
+  if (p.reconstruction != ZFV_CWENO_AO && p.reconstruction != ZFV_WENO_AO)
+    throw ZfvError(ZFV_ERR_INVALID, "reconstruction must be ZFV_CWENO_AO (0) or ZFV_WENO_AO (1)");
+  if (p.flux != ZFV_FLUX_HLLC && p.flux != ZFV_FLUX_RUSANOV)
+    throw ZfvError(ZFV_ERR_INVALID, "flux must be ZFV_FLUX_HLLC (0) or ZFV_FLUX_RUSANOV (1)");
+  if (p.equilibrium < ZFV_EQ_NONE || p.equilibrium > ZFV_EQ_ISOTHERMAL)
+    throw ZfvError(ZFV_ERR_INVALID, "equilibrium must be ZFV_EQ_NONE, ZFV_EQ_ISENTROPIC or ZFV_EQ_ISOTHERMAL");
+  if (p.gravity < ZFV_GRAVITY_NONE || p.gravity > ZFV_GRAVITY_POLYTROPE)
+    throw ZfvError(ZFV_ERR_INVALID, "gravity must be one of ZFV_GRAVITY_NONE/AXIAL/RADIAL/POLYTROPE");
+  if (p.flux_bc < ZFV_FLUX_BC_NONE || p.flux_bc > ZFV_FLUX_BC_ISENTROPIC)
+    throw ZfvError(ZFV_ERR_INVALID, "flux_bc must be ZFV_FLUX_BC_NONE (0), _CONSTANT (1) or _ISENTROPIC (2)");
+  if (p.flux_bc == ZFV_FLUX_BC_ISENTROPIC && p.equilibrium != ZFV_EQ_ISENTROPIC)
+    throw ZfvError(ZFV_ERR_INVALID,
+                   "flux_bc isentropic (EquilibriumFluxBC) is implemented with isentropic well-balancing only");
+  if (!(p.gamma > 1.0)) throw ZfvError(ZFV_ERR_INVALID, "gamma must be > 1");
   if (p.scaling != 0) throw ZfvError(ZFV_ERR_INVALID, "the kernels implement EulerScaling only (scaling = 0)");
   if (p.steps_per_recompute != 1)
     throw ZfvError(ZFV_ERR_INVALID, "steps_per_recompute must be 1: the kernels re-solve equilibria every call");

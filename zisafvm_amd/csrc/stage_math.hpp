@@ -1089,6 +1089,16 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 
   double s_acc = 0.0;
   const bool gravity_src = (m.gravity != 0) && (src != nullptr);
+  // EquilibriumFluxBC (flux_bc 2, include/zisa/boundary/equilibrium_flux_bc.hpp:36-62):
+  // at the cell's boundary faces, the flux of its isentropic equilibrium
+  // (this cell's own, solved above: the same solve_exact from the same
+  // average) as a state at rest, (0, p_eq n, 0), integrated with the face
+  // rule and subtracted.  It has the form of the well-balanced source's face
+  // term, w p_eq (sgn n), so it enters that sum as w p_eq ((sgn - 1) n): the
+  // two cancel at a face whose frame normal points out of the cell.  An
+  // equilibrium that was not found gives NaN, as the reference's zero
+  // extrapolation does.  (Isentropic only: never in the light physics set.)
+  const bool eq_flux_bc = PH != PH_LIGHT && m.flux_bc == 2 && gravity_src;
 
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -1132,6 +1142,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       }
     }
     double fs = 0.0;
+    const double bc_sub = (eq_flux_bc && cf == 2) ? 1.0 : 0.0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       if (!exterior) {
@@ -1153,11 +1164,12 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       if (gravity_src && eq_sub && g >= 1 && g <= DIM) {
         double p_eq = eqE[q] * (gamma - 1.0);
         double w = area * m.fw[q];
-        double t = p_eq * (sgn * nn[g - 1]);
+        double t = p_eq * ((sgn - bc_sub) * nn[g - 1]);
         fs = (q == 0) ? w * t : fs + w * t;
       }
     }
     if (gravity_src && wb) s_acc += 1.0 * fs;
+    if (eq_flux_bc && cf == 2 && !eq_sub) s_acc = __builtin_nan("");
   }
 
   if (gravity_src) {
@@ -1249,8 +1261,9 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
   for (int f = 0; f < NF; ++f) {
     const int32_t cf = m.cface[f * n + i];
     const bool exterior = (cf & 2) != 0;
-    // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC
-    if (exterior && (m.flux_bc == 0 || (cf & 1))) continue;
+    // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC; the
+    // equilibrium flux BC (flux_bc 2) is part of the source term
+    if (exterior && (m.flux_bc != 1 || (cf & 1))) continue;
     double nn[3], t1[3], t2[3], area;
     load_face_frame<DIM>(m, i, f, nn, t1, t2, area);
     double g[NV];

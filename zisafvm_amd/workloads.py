@@ -154,6 +154,34 @@ def isothermal_2d(n=360, order=4, amplitude=1e-6, seed=2, jitter=0.2, ghost_laye
                     meta=dict(n=n, jitter=jitter, seed=seed, amplitude=amplitude, order=order))
 
 
+def isentropic_atmosphere(x, g=1.0, K=1.0, gamma=1.4, h0=3.5, amplitude=0.0, width=0.1, center=(0.5, 0.5)):
+    """Isentropic atmosphere under gravity phi = g y: enthalpy h = h0 - g y,
+    rho = ((gamma-1)/(gamma K) h)^(1/(gamma-1)), p = K rho^gamma
+    (include/zisa/model/isentropic_equilibrium.hpp:44-63,
+    include/zisa/model/ideal_gas_eos.hpp:162-168), with a Gaussian pressure bump."""
+    y = x[:, 1]
+    rho = ((gamma - 1.0) / (gamma * K) * (h0 - g * y)) ** (1.0 / (gamma - 1.0))
+    r2 = ((x[:, 0] - center[0]) ** 2 + (x[:, 1] - center[1]) ** 2) / width ** 2
+    p = K * rho ** gamma * (1.0 + amplitude * np.exp(-r2))
+    zz = 0.0 * y
+    return conserved(rho, zz, zz, zz, p, gamma)
+
+
+def isentropic_2d(n=16, order=3, amplitude=0.0, gamma=1.4, seed=6, jitter=0.2, flux_bc=2, ghost_layers=0) -> Workload:
+    """Well-balanced 2D with the reference's isentropic equilibrium for a
+    general gamma (its per-point power laws): periodic in x, solid-wall rows at
+    y = 0 and y = 1 closed by the equilibrium flux BC (flux_bc 2) unless
+    ghost layers are asked for."""
+    mesh = lattice_mesh_2d(n, jitter=jitter, seed=seed, periodic=(True, False), ghost_layers=ghost_layers)
+    p = make_params(n_dims=2, order=order, flux=ZFV_FLUX_HLLC, equilibrium=ZFV_EQ_ISENTROPIC,
+                    gravity=ZFV_GRAVITY_AXIAL, gravity_g=1.0, gravity_axis=(0.0, 1.0, 0.0), gamma=gamma,
+                    flux_bc=flux_bc)
+    u0 = cell_average(mesh, lambda x: isentropic_atmosphere(x, gamma=gamma, amplitude=amplitude))
+    steady = cell_average(mesh, lambda x: isentropic_atmosphere(x, gamma=gamma))
+    return Workload("isentropic2d", mesh, p, u0, steady if ghost_layers > 0 else None, t_end=0.1, cfl=0.45,
+                    meta=dict(n=n, jitter=jitter, seed=seed, amplitude=amplitude, order=order, gamma=gamma))
+
+
 def batched_2d(n_instances=64, min_cells=5000, max_cells=200000, seed=4, order=3):
     """Batched workload: instances with triangle counts log-uniform in
     [min_cells, max_cells]; even members are isentropic vortices (strength
