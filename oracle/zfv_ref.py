@@ -55,6 +55,12 @@ def load():
         "zfv_ref_rusanov": (c_int, [c_double, dp, dp, dp]),
         "zfv_ref_rule": (c_int, [c_int, c_int, dp, dp, c_int]),
         "zfv_ref_last_error": (ctypes.c_char_p, []),
+        "zfv_ref_kat_error": (ctypes.c_char_p, []),
+        "zfv_ref_eq_solve": (c_int, [vp, c_int, dp, c_int, c_double, c_double, dp, dp, ctypes.POINTER(c_int)]),
+        "zfv_ref_eq_extrapolate": (c_int, [vp, dp, dp, c_int, dp, dp]),
+        "zfv_ref_eq_average": (c_int, [vp, dp, dp, c_int, dp, c_int, dp]),
+        "zfv_ref_poly_eval": (c_int, [c_int, c_int, dp, dp, dp, c_double, dp, dp]),
+        "zfv_ref_poly_lincomb": (c_int, [c_int, ip, ip, dp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -200,3 +206,107 @@ def rk_ode(kind, dt, t_end):
     out = ctypes.c_double()
     lib.zfv_ref_rk_ode(kind, dt, t_end, ctypes.byref(out))
     return out.value
+
+
+# ---- pieces for the reference's unit-test known answers -----------------------
+_QN_F = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+_QN_IDF = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double))
+
+
+def quasi_newton(f, inv_df, x0, atol, max_iter=20):
+    """The oracle's quasi_newton on f: R^n -> R^n (n = 1, 2) with the given
+    inverse Jacobian inv_df(x, fx) -> dx; returns (x, status, steps) with
+    status 0 converged, 1 not converged (x = x0), 2 not converging."""
+    n = len(x0)
+
+    def cf(x, fx):
+        r = f(np.array([x[k] for k in range(n)]))
+        for k in range(n):
+            fx[k] = r[k]
+
+    def cidf(x, fx, dx):
+        r = inv_df(np.array([x[k] for k in range(n)]), np.array([fx[k] for k in range(n)]))
+        for k in range(n):
+            dx[k] = r[k]
+
+    lib = load()
+    lib.zfv_ref_quasi_newton.argtypes = [ctypes.c_int, _QN_F, _QN_IDF, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    fcb, icb = _QN_F(cf), _QN_IDF(cidf)
+    x = np.zeros(n)
+    steps = ctypes.c_int()
+    st = lib.zfv_ref_quasi_newton(n, fcb, icb, _dp(np.array(x0, float)), _dp(np.array(atol, float)), max_iter,
+                                  _dp(x), ctypes.byref(steps))
+    return x, st, steps.value
+
+
+def _p(params):
+    from zisafvm_amd import Params  # struct layout only (include/zfv.h)
+
+    return ctypes.byref(params) if isinstance(params, Params) else params
+
+
+def eq_solve(params, corners, deg, rho_bar, E_bar):
+    """LocalEquilibrium::solve on one simplex (corners: (d+1, 3)); returns
+    (theta, x_ref, found)."""
+    lib = load()
+    c = np.ascontiguousarray(corners, float)
+    th, xr, found = np.zeros(2), np.zeros(3), ctypes.c_int()
+    rc = lib.zfv_ref_eq_solve(_p(params), c.shape[0] - 1, _dp(c), deg, ctypes.c_double(rho_bar),
+                              ctypes.c_double(E_bar), _dp(th), _dp(xr), ctypes.byref(found))
+    if rc != 0:
+        raise RuntimeError(lib.zfv_ref_kat_error().decode())
+    return th, xr, bool(found.value)
+
+
+def eq_extrapolate(params, theta, x_ref, x):
+    x = np.ascontiguousarray(np.atleast_2d(x), float)
+    out = np.zeros((x.shape[0], 2))
+    load().zfv_ref_eq_extrapolate(_p(params), _dp(np.array(theta, float)), _dp(np.array(x_ref, float)),
+                                  x.shape[0], _dp(x), _dp(out))
+    return out
+
+
+def eq_average(params, theta, x_ref, corners, deg):
+    c = np.ascontiguousarray(corners, float)
+    out = np.zeros(2)
+    load().zfv_ref_eq_average(_p(params), _dp(np.array(theta, float)), _dp(np.array(x_ref, float)),
+                              c.shape[0] - 1, _dp(c), deg, _dp(out))
+    return out
+
+
+def poly_eval(n_dims, degree, coeffs, moments, x, center=(0.0, 0.0, 0.0), length=1.0):
+    """PolyND evaluation; coeffs (dof, nv<=5), moments (dof,)."""
+    coeffs = np.atleast_2d(np.asarray(coeffs, float))
+    if coeffs.shape[0] == 1 and coeffs.shape[1] > 5:
+        coeffs = coeffs.T
+    dof = coeffs.shape[0]
+    a = np.zeros((dof, 5))
+    a[:, : coeffs.shape[1]] = coeffs
+    out = np.zeros(5)
+    load().zfv_ref_poly_eval(n_dims, degree, _dp(a), _dp(np.asarray(moments, float)),
+                             _dp(np.asarray(center, float)), ctypes.c_double(length),
+                             _dp(np.asarray(x, float)), _dp(out))
+    return out
+
+
+def poly_lincomb(polys, weights, x):
+    """sum_k w_k p_k formed as the WENO sum is formed (metadata of the larger
+    degree, ties to the right operand), evaluated at x.  polys: list of
+    (n_dims, degree, coeffs (dof, nv), moments (dof,))."""
+    maxc = 35
+    nd = np.array([p[0] for p in polys], dtype=np.int32)
+    dg = np.array([p[1] for p in polys], dtype=np.int32)
+    a = np.zeros((len(polys), maxc, 5))
+    c = np.zeros((len(polys), maxc))
+    for j, (_, _, co, mo) in enumerate(polys):
+        co = np.atleast_2d(np.asarray(co, float))
+        a[j, : co.shape[0], : co.shape[1]] = co
+        c[j, : len(mo)] = mo
+    out = np.zeros(5)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    load().zfv_ref_poly_lincomb(len(polys), nd.ctypes.data_as(ip), dg.ctypes.data_as(ip), _dp(a), _dp(c),
+                                _dp(np.asarray(weights, float)), _dp(np.asarray(x, float)), _dp(out))
+    return out

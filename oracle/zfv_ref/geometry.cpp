@@ -202,6 +202,11 @@ static DenormalizedRule make_cell(int n_dims, const std::vector<XYZ> &v, int deg
   return denormalize_simplex(tetrahedral_rule(deg), v, tet_volume(v[0], v[1], v[2], v[3]));
 }
 
+// make_cell (src/zisa/math/cell_factory.cpp) of one free-standing simplex
+DenormalizedRule simplex_cell_rule(int n_dims, const XYZ *v, int deg) {
+  return make_cell(n_dims, std::vector<XYZ>(v, v + n_dims + 1), deg);
+}
+
 static double ipow(double x, int k) {
   double r = 1.0;
   for (int i = 0; i < k; ++i) r *= x;

@@ -3,8 +3,10 @@
 // change (FluxLoop + GravitySourceLoop + FluxBC), SSP3 and CFL.
 // TEST INFRASTRUCTURE ONLY (see zfv_ref.hpp).
 #include "zfv_ref.hpp"
+#include "../../include/zfv.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #ifdef _OPENMP
 #include <omp.h>
@@ -150,6 +152,62 @@ static auto cell_average(const DenormalizedRule &qr, const F &f) {
   return ret;
 }
 
+// quasi_newton (include/zisa/math/quasi_newton.hpp:12-50) with the
+// RollingConvergenceRate of its steps (include/zisa/math/rolling_convergence_rate.hpp):
+// x <- x - inv_df(x, f(x)) until every |dx_k| <= atol_k or max_iter steps.
+// From the fifth step on, unless the last step is within atol (signed, as
+// the reference compares it), every component's rate
+// log(|c|/|b|) / log(|b|/|a|) over the last three steps must be >= 0, else
+// the solve is "not converging" (the reference aborts with LOG_ERR).  After
+// max_iter steps without every |dx_k| <= 1000 atol_k: not converged, and x0
+// is returned, as the reference does.
+enum QNStatus { QN_CONVERGED = 0, QN_NOT_CONVERGED = 1, QN_NOT_CONVERGING = 2 };
+
+template <int N, class F, class IDF>
+QNStatus quasi_newton(const F &f, const IDF &inv_df, const std::array<double, N> &x0,
+                      const std::array<double, N> &atol, int max_iter, std::array<double, N> &x,
+                      std::vector<std::array<double, N>> *steps = nullptr) {
+  x = x0;
+  auto fx = f(x);
+  std::array<double, N> dx;
+  for (int k = 0; k < N; ++k) dx[k] = 2.0 * atol[k] + 1.0;
+  auto converged = [&](double factor) {
+    for (int k = 0; k < N; ++k)
+      if (!(std::abs(dx[k]) <= factor * atol[k])) return false;
+    return true;
+  };
+  std::vector<std::array<double, N>> hist;
+  int iter = 0;
+  while (!converged(1.0) && iter < max_iter) {
+    dx = inv_df(x, fx);
+    for (int k = 0; k < N; ++k) x[k] -= dx[k];
+    fx = f(x);
+    hist.push_back(dx);
+    if (steps) steps->push_back(dx);
+    if (iter >= 4) {
+      const auto &c = hist[hist.size() - 1];
+      bool within = true;
+      for (int k = 0; k < N; ++k) within = within && c[k] <= atol[k];
+      if (!within) {
+        const auto &a = hist[hist.size() - 3], &b = hist[hist.size() - 2];
+        for (int k = 0; k < N; ++k) {
+          double rate = std::log(std::abs(c[k]) / std::abs(b[k])) / std::log(std::abs(b[k]) / std::abs(a[k]));
+          if (!(rate >= 0.0)) {
+            x = x0;
+            return QN_NOT_CONVERGING;
+          }
+        }
+      }
+    }
+    ++iter;
+  }
+  if (iter == max_iter && !converged(1000.0)) {
+    x = x0;
+    return QN_NOT_CONVERGED;
+  }
+  return QN_CONVERGED;
+}
+
 struct LocalEquilibrium {
   int kind = 0;  // 0 none, 1 isentropic, 2 isothermal
   const EOS *eos = nullptr;
@@ -187,74 +245,45 @@ struct LocalEquilibrium {
       found = true;
       return;
     }
-    // isentropic: quasi-Newton on (h, K) (include/zisa/math/quasi_newton.hpp:12-50)
+    // isentropic: quasi-Newton on (h, K) with a central finite-difference
+    // Jacobian (include/zisa/model/local_equilibrium_impl.hpp:41-93)
     double p = bar.E * (gamma - 1.0);
-    double guess[2] = {gamma / (gamma - 1.0) * p / bar.rho, p / std::pow(bar.rho, gamma)};
-    auto f = [&](const double *th) {
-      auto avg = cell_average(cell, [&](const XYZ &x) { return extrapolate_with(th, x); });
+    std::array<double, 2> guess{gamma / (gamma - 1.0) * p / bar.rho, p / std::pow(bar.rho, gamma)};
+    auto f = [&](const std::array<double, 2> &th) {
+      auto avg = cell_average(cell, [&](const XYZ &x) { return extrapolate_with(th.data(), x); });
       return std::array<double, 2>{bar.rho - avg[0], bar.E - avg[1]};
     };
-    auto df = [&](const double *x, int dir) {
+    auto df = [&](const std::array<double, 2> &x, int dir) {
       double eps = 1e-6 * std::abs(x[dir]);
-      double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]};
+      std::array<double, 2> xp = x, xm = x;
       xp[dir] = x[dir] + 0.5 * eps;
       xm[dir] = x[dir] - 0.5 * eps;
       auto fp = f(xp), fm = f(xm);
       return std::array<double, 2>{(fp[0] - fm[0]) / eps, (fp[1] - fm[1]) / eps};
     };
-    double atol[2] = {1e-13 * guess[0], 1e-13 * guess[1]};
-    double x[2] = {guess[0], guess[1]};
-    auto fx = f(x);
-    double dx[2] = {2.0 * atol[0] + 1.0, 2.0 * atol[1] + 1.0};
-    auto converged = [&](double factor) {
-      return std::abs(dx[0]) <= factor * atol[0] && std::abs(dx[1]) <= factor * atol[1];
-    };
-    // RollingConvergenceRate (include/zisa/math/rolling_convergence_rate.hpp)
-    std::vector<std::array<double, 2>> hist;
-    int iter = 0;
-    const int max_iter = 20;
-    bool ok = true;
-    while (!converged(1.0) && iter < max_iter) {
+    auto inv_df = [&](const std::array<double, 2> &x, const std::array<double, 2> &fx) {
       auto d0 = df(x, 0), d1 = df(x, 1);
       double inv_det = 1.0 / (d0[0] * d1[1] - d0[1] * d1[0]);
-      dx[0] = inv_det * (d1[1] * fx[0] - d1[0] * fx[1]);
-      dx[1] = inv_det * (-d0[1] * fx[0] + d0[0] * fx[1]);
-      x[0] -= dx[0];
-      x[1] -= dx[1];
-      fx = f(x);
-      hist.push_back({dx[0], dx[1]});
-      if (iter >= 4) {
-        const auto &c = hist[hist.size() - 1];
-        bool conv = c[0] <= atol[0] && c[1] <= atol[1];
-        if (!conv) {
-          const auto &a = hist[hist.size() - 3], &b = hist[hist.size() - 2];
-          for (int k = 0; k < 2; ++k) {
-            double rate = std::log(std::abs(c[k]) / std::abs(b[k])) / std::log(std::abs(b[k]) / std::abs(a[k]));
-            if (!(rate >= 0.0)) ok = false;
-          }
-        }
-        if (!ok) {
-          std::string msg = "quasi_newton: Is not converging. guess=(" + std::to_string(guess[0]) + "," +
-                            std::to_string(guess[1]) + ") dx:";
-          char buf[64];
-          for (const auto &h : hist) {
-            snprintf(buf, sizeof(buf), " (%.3e,%.3e)", h[0], h[1]);
-            msg += buf;
-          }
-          throw Error(msg);
-        }
+      return std::array<double, 2>{inv_det * (d1[1] * fx[0] - d1[0] * fx[1]),
+                                   inv_det * (-d0[1] * fx[0] + d0[0] * fx[1])};
+    };
+    std::array<double, 2> atol{1e-13 * guess[0], 1e-13 * guess[1]};
+    std::array<double, 2> x;
+    std::vector<std::array<double, 2>> hist;
+    const QNStatus st = quasi_newton<2>(f, inv_df, guess, atol, 20, x, &hist);
+    if (st == QN_NOT_CONVERGING) {
+      std::string msg = "quasi_newton: Is not converging. guess=(" + std::to_string(guess[0]) + "," +
+                        std::to_string(guess[1]) + ") dx:";
+      char buf[64];
+      for (const auto &h : hist) {
+        snprintf(buf, sizeof(buf), " (%.3e,%.3e)", h[0], h[1]);
+        msg += buf;
       }
-      ++iter;
-    }
-    if (iter == max_iter && !converged(1000.0)) {
-      found = false;
-      theta[0] = guess[0];
-      theta[1] = guess[1];
-      return;
+      throw Error(msg);
     }
     theta[0] = x[0];
     theta[1] = x[1];
-    found = true;
+    found = st == QN_CONVERGED;
   }
 
   RhoE extrapolate(const XYZ &x) const {
@@ -842,3 +871,178 @@ double Solver::cfl_dt(const double *u, double cfl) const {
 namespace zfv_ref {
 void Solver::set_threads(int n) { impl_->n_threads = n < 1 ? 1 : n; }
 }  // namespace zfv_ref
+
+// ---------------------------------------------------------------------------
+// Entry points of the oracle's pieces for the known-answer tests of the
+// reference's unit tests (test/zisa/unit_test/math/{quasi_newton,poly2d}.cpp,
+// test/zisa/unit_test/model/local_equilibrium.cpp).  Test infrastructure.
+namespace {
+zfv_ref::EOS kat_eos(const zfv_params *p) {
+  zfv_ref::EOS e;
+  e.gamma = p->gamma;
+  e.R = p->gas_constant;
+  return e;
+}
+zfv_ref::Gravity kat_gravity(const zfv_params *p) {
+  zfv_ref::Gravity g;
+  g.kind = p->gravity;
+  g.g = p->gravity_g;
+  g.axis = {p->gravity_axis[0], p->gravity_axis[1], p->gravity_axis[2]};
+  g.rhoC = p->polytrope_rhoC;
+  g.K = p->polytrope_K;
+  g.G = p->polytrope_G;
+  return g;
+}
+zfv_ref::DenormalizedRule kat_cell(int n_dims, const double *corners, int deg) {
+  zfv_ref::XYZ v[4];
+  for (int k = 0; k <= n_dims; ++k) v[k] = {corners[3 * k], corners[3 * k + 1], corners[3 * k + 2]};
+  return zfv_ref::simplex_cell_rule(n_dims, v, deg);
+}
+thread_local std::string kat_err;
+}  // namespace
+
+extern "C" {
+
+const char *zfv_ref_kat_error(void) { return kat_err.c_str(); }
+
+// quasi_newton on an n-dimensional system given by callbacks:
+// f(x, fx) and inv_df(x, fx, dx).  Returns the status (0 converged,
+// 1 not converged, 2 not converging) and the result in x.
+typedef void (*zfv_ref_qn_f)(const double *x, double *fx);
+typedef void (*zfv_ref_qn_idf)(const double *x, const double *fx, double *dx);
+int zfv_ref_quasi_newton(int n, zfv_ref_qn_f f, zfv_ref_qn_idf inv_df, const double *x0, const double *atol,
+                         int max_iter, double *x, int *n_steps) {
+  auto run = [&](auto tag) {
+    constexpr int N = decltype(tag)::value;
+    std::array<double, N> a0, at, xr;
+    for (int k = 0; k < N; ++k) {
+      a0[k] = x0[k];
+      at[k] = atol[k];
+    }
+    auto ff = [&](const std::array<double, N> &y) {
+      std::array<double, N> r;
+      f(y.data(), r.data());
+      return r;
+    };
+    auto idf = [&](const std::array<double, N> &y, const std::array<double, N> &fy) {
+      std::array<double, N> d;
+      inv_df(y.data(), fy.data(), d.data());
+      return d;
+    };
+    std::vector<std::array<double, N>> steps;
+    int st = zfv_ref::quasi_newton<N>(ff, idf, a0, at, max_iter, xr, &steps);
+    for (int k = 0; k < N; ++k) x[k] = xr[k];
+    if (n_steps) *n_steps = int(steps.size());
+    return st;
+  };
+  if (n == 1) return run(std::integral_constant<int, 1>{});
+  if (n == 2) return run(std::integral_constant<int, 2>{});
+  kat_err = "quasi_newton: n must be 1 or 2";
+  return -1;
+}
+
+// LocalEquilibrium::solve (solve_exact) of p->equilibrium on one simplex
+// (n_dims + 1 corners, xyz each) with its cell rule of degree deg.
+int zfv_ref_eq_solve(const zfv_params *p, int n_dims, const double *corners, int deg, double rho_bar,
+                     double E_bar, double *theta, double *x_ref, int *found) {
+  try {
+    zfv_ref::EOS eos = kat_eos(p);
+    zfv_ref::Gravity grav = kat_gravity(p);
+    zfv_ref::LocalEquilibrium eq;
+    eq.kind = p->equilibrium;
+    eq.eos = &eos;
+    eq.gravity = &grav;
+    eq.solve(zfv_ref::RhoE{rho_bar, E_bar}, kat_cell(n_dims, corners, deg));
+    theta[0] = eq.theta[0];
+    theta[1] = eq.theta[1];
+    for (int d = 0; d < 3; ++d) x_ref[d] = eq.x_ref[d];
+    *found = eq.found ? 1 : 0;
+    return 0;
+  } catch (const std::exception &e) {
+    kat_err = e.what();
+    return -1;
+  }
+}
+
+// Equilibrium.extrapolate(theta, x_ref, x) at n points (rho, E per point)
+int zfv_ref_eq_extrapolate(const zfv_params *p, const double *theta, const double *x_ref, int n, const double *x,
+                           double *rhoE) {
+  zfv_ref::EOS eos = kat_eos(p);
+  zfv_ref::Gravity grav = kat_gravity(p);
+  zfv_ref::LocalEquilibrium eq;
+  eq.kind = p->equilibrium;
+  eq.eos = &eos;
+  eq.gravity = &grav;
+  eq.theta[0] = theta[0];
+  eq.theta[1] = theta[1];
+  eq.x_ref = {x_ref[0], x_ref[1], x_ref[2]};
+  eq.found = true;
+  for (int q = 0; q < n; ++q) {
+    auto r = eq.extrapolate({x[3 * q], x[3 * q + 1], x[3 * q + 2]});
+    rhoE[2 * q] = r.rho;
+    rhoE[2 * q + 1] = r.E;
+  }
+  return 0;
+}
+
+// average(cell, extrapolate) over one simplex with its cell rule of degree deg
+int zfv_ref_eq_average(const zfv_params *p, const double *theta, const double *x_ref, int n_dims,
+                       const double *corners, int deg, double *rhoE) {
+  zfv_ref::EOS eos = kat_eos(p);
+  zfv_ref::Gravity grav = kat_gravity(p);
+  zfv_ref::LocalEquilibrium eq;
+  eq.kind = p->equilibrium;
+  eq.eos = &eos;
+  eq.gravity = &grav;
+  eq.theta[0] = theta[0];
+  eq.theta[1] = theta[1];
+  eq.x_ref = {x_ref[0], x_ref[1], x_ref[2]};
+  eq.found = true;
+  auto r = eq.extrapolate_cell(kat_cell(n_dims, corners, deg));
+  rhoE[0] = r.rho;
+  rhoE[1] = r.E;
+  return 0;
+}
+
+// PolyND evaluation: coefficients a[alpha][var] (dof x 5), moments c[alpha]
+int zfv_ref_poly_eval(int n_dims, int degree, const double *coeffs, const double *moments, const double *center,
+                      double length, const double *x, double *out) {
+  zfv_ref::Poly p;
+  p.n_dims = n_dims;
+  p.degree = degree;
+  const int dof = zfv_ref::poly_dof(degree, n_dims);
+  for (int i = 0; i < dof * zfv_ref::NVARS; ++i) p.a[i] = coeffs[i];
+  for (int i = 0; i < dof; ++i) p.c[i] = moments[i];
+  p.center = {center[0], center[1], center[2]};
+  p.length = length;
+  auto v = p({x[0], x[1], x[2]});
+  for (int k = 0; k < zfv_ref::NVARS; ++k) out[k] = v[k];
+  return 0;
+}
+
+// sum_k w_k p_k formed as the reconstruction forms its WENO sum (starting
+// from the zero polynomial of degree 0 in 2D), each partial sum taking the
+// metadata of the larger-degree operand, ties to the right one; evaluated at x
+int zfv_ref_poly_lincomb(int n_polys, const int *n_dims, const int *degrees, const double *coeffs,
+                         const double *moments, const double *weights, const double *x, double *out) {
+  zfv_ref::Poly acc;
+  acc.degree = 0;
+  acc.n_dims = 2;
+  for (int j = 0; j < n_polys; ++j) {
+    zfv_ref::Poly p;
+    p.n_dims = n_dims[j];
+    p.degree = degrees[j];
+    const int dof = zfv_ref::poly_dof(degrees[j], n_dims[j]);
+    for (int i = 0; i < dof * zfv_ref::NVARS; ++i) p.a[i] = coeffs[j * zfv_ref::MAX_COEFFS * zfv_ref::NVARS + i];
+    for (int i = 0; i < dof; ++i) p.c[i] = moments[j * zfv_ref::MAX_COEFFS + i];
+    zfv_ref::Poly t;
+    zfv_ref::binary_meta(t, acc, p);
+    for (int q = 0; q < zfv_ref::MAX_COEFFS * zfv_ref::NVARS; ++q) t.a[q] = acc.a[q] + weights[j] * p.a[q];
+    acc = t;
+  }
+  auto v = acc({x[0], x[1], x[2]});
+  for (int k = 0; k < zfv_ref::NVARS; ++k) out[k] = v[k];
+  return 0;
+}
+
+}  // extern "C"

@@ -101,6 +101,9 @@ struct DenormalizedRule {
   std::vector<XYZ> points;
   double volume = 0.0;
 };
+// the cell rule of one simplex with n_dims + 1 corners
+DenormalizedRule simplex_cell_rule(int n_dims, const XYZ *v, int deg);
+
 
 struct Face {
   DenormalizedRule qr;

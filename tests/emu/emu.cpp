@@ -340,6 +340,23 @@ int zfv_emu_validate(void *h, char *msg, int cap) {
   return 1;
 }
 
+// The kernels' polynomial evaluation (poly_eval of one variable) at the
+// scaled point X = (x - center) / length: coefficients a[alpha], moments c[alpha].
+int zfv_emu_poly_eval(int n_dims, int degree, const double *a, const double *c, const double *X, double *out) {
+  const double(*aa)[1] = reinterpret_cast<const double(*)[1]>(a);
+  double px = 0.0;
+  if (n_dims == 2 && degree == 1) poly_eval<2, 1, 1>(aa, c, X, &px);
+  else if (n_dims == 2 && degree == 2) poly_eval<2, 2, 1>(aa, c, X, &px);
+  else if (n_dims == 2 && degree == 3) poly_eval<2, 3, 1>(aa, c, X, &px);
+  else if (n_dims == 2 && degree == 4) poly_eval<2, 4, 1>(aa, c, X, &px);
+  else if (n_dims == 3 && degree == 1) poly_eval<3, 1, 1>(aa, c, X, &px);
+  else if (n_dims == 3 && degree == 2) poly_eval<3, 2, 1>(aa, c, X, &px);
+  else if (n_dims == 3 && degree == 3) poly_eval<3, 3, 1>(aa, c, X, &px);
+  else return -1;
+  *out = px;
+  return 0;
+}
+
 int zfv_emu_info(void *h, int64_t *n, int64_t *ne, double *max_cond) {
   const HostImage &im = static_cast<Emu *>(h)->img;
   *n = im.n;
