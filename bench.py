@@ -9,12 +9,14 @@ rho = p = exp(-y) with a 1e-6 Gaussian pressure bump on a seeded jittered
 frozen ghost layers), CWENO-AO order 4, HLLC, SSP3, CFL 0.45.
 
 One step = one SSP3 step (3 stages) of the whole instance, replayed from a
-captured HIP graph with the state resident in HBM.  value = n_cells * 3 * K / T
-over exactly K timed steps.  The JSON line also carries
+captured HIP graph with the state resident in HBM.  value = n_interior * 3 * K
+/ T over exactly K timed steps, counting the interior cells (the
+configuration's triangles or tetrahedra; the frozen ghost layers are not
+counted).  The JSON line also carries
   - roofline: the dominant kernel (k_reconstruct), algorithmic bytes per launch
     / its average HIP-event duration on the handle's stream;
   - cpu_baseline: the oracle (the reference's OpenMP CPU path restated) timed
-    on this host's cores over a bounded sample.
+    on all of this host's cores over a bounded sample, and on one thread.
 Opt-in legs:
   --batch  batched_vs_plain: the batched configuration (64 members, 5k-200k
            cells, order 3, per-member CFL every step, one graph per step for
@@ -43,11 +45,23 @@ def log(msg):
 
 
 def cpu_cores():
+    """Host threads this process may run on (its affinity), all of them."""
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def build_workload(name, n):
@@ -74,28 +88,39 @@ def measured_traffic(workload, n_cells):
         return None
 
 
-def cpu_baseline(wl, budget_s):
-    """Oracle rate-of-change calls on this host's cores for about budget_s."""
+def cpu_baseline(wl, budget_s, n_counted):
+    """Oracle rate-of-change calls (one RK stage each) on all of this host's
+    cores for about budget_s, then on one thread for a shorter sample;
+    cell-stage updates/s counted over the same cells as the GPU value."""
     from oracle.zfv_ref import Oracle
 
     cores = cpu_cores()
     o = Oracle(wl.mesh, wl.params, threads=cores)
-    t0 = time.perf_counter()
-    o.rate(wl.u0)
-    first = time.perf_counter() - t0
-    calls = int(min(20, max(1, budget_s / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(calls):
+
+    def timed(threads, budget, max_calls):
+        o.set_threads(threads)
+        t0 = time.perf_counter()
         o.rate(wl.u0)
-    el = time.perf_counter() - t0
+        first = time.perf_counter() - t0
+        calls = int(min(max_calls, max(1, budget / max(first, 1e-3))))
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            o.rate(wl.u0)
+        return calls, time.perf_counter() - t0
+
+    calls, el = timed(cores, budget_s, 20)
+    calls1, el1 = timed(1, budget_s / 3.0, 5)
     n = wl.mesh.n_cells
     return {
-        "value": n * calls / el,
+        "value": n_counted * calls / el,
         "unit": "cell-stage updates/s",
         "cores": cores,
         "kind": "port",
+        "cpu_model": cpu_model(),
+        "single_thread_value": n_counted * calls1 / el1,
         "sample": f"{calls} rate-of-change evaluations (one RK stage each) of the {wl.name} instance "
-        f"({n} cells) by the oracle, OpenMP on {cores} threads, {el:.1f} s",
+        f"({n} cells, {n_counted} counted) by the oracle, OpenMP on all {cores} host threads, {el:.1f} s; "
+        f"single thread: {calls1} evaluations, {el1:.1f} s",
     }
 
 
@@ -238,8 +263,13 @@ def main():
     log(f"workload {wl.name}: {wl.mesh.n_cells} cells, mesh+IC {time.perf_counter() - t0:.1f}s")
     s, info, dt, el, t_setup = run_single(z, wl, args.steps, args.warmup)
     n = wl.mesh.n_cells
-    value = n * 3 * args.steps / el
-    log(f"setup {t_setup:.1f}s; {args.steps} steps in {el:.3f}s: {value:.3e} cell-stage updates/s")
+    # cell-stage updates are counted over the interior cells, whose states
+    # the steps advance; ghost cells are held at the steady state by the
+    # frozen BC (those away from the interior are not even reconstructed)
+    n_interior = int((wl.mesh.ghost == 0).sum())
+    value = n_interior * 3 * args.steps / el
+    log(f"setup {t_setup:.1f}s; {args.steps} steps in {el:.3f}s: {value:.3e} cell-stage updates/s "
+        f"({n_interior} interior cells of {n})")
     # per-kernel HIP-event times right after the timed steps, with the GPU
     # still at the clocks the steps ran at
     kt = s.time_kernels(max(2, min(10, args.steps)), dt)
@@ -298,7 +328,8 @@ def main():
         "config": {
             "workload": wl.name,
             "n_cells": n,
-            "n_interior_cells": int((wl.mesh.ghost == 0).sum()),
+            "n_interior_cells": n_interior,
+            "value_counts": "interior cells (the configuration's triangles / tetrahedra); ghost layers excluded",
             "n_reconstructed_cells": n_rec,
             "order": wl.meta.get("order"),
             "mesh": wl.meta,
@@ -319,18 +350,19 @@ def main():
             "cells_per_launch": n_rec,
             "kernel": "k_reconstruct",
             "bytes_per_cell": info.recon_bytes_per_cell,
-            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE of this workload, "
-                              "scaled by the copy-kernel calibration)",
+            "traffic_source": "copied from profiles/traffic.json, not measured in this run: rocprofv3 "
+                              "FETCH_SIZE/WRITE_SIZE of this workload's k_reconstruct, each scaled by what the "
+                              "counter reads for a copy kernel of known size",
         },
         "setup_s": t_setup,
         "device_time_loop": {"steps": adv_steps, "ms_per_step": 1e3 * el_adv / max(adv_steps, 1),
-                             "cell_stage_per_s": n * 3 * adv_steps / el_adv,
+                             "cell_stage_per_s": n_interior * 3 * adv_steps / el_adv,
                              "note": "zfv_advance: CFL dt chosen on the device every step"},
         "snapshot_overlap": {"ms_per_step": 1e3 * el_snap / args.steps, "state_mb": n * 5 * 8 / 1e6,
                              "end_ms": snap_end_ms,
                              "note": "the fixed-dt steps with a snapshot copied to the host behind them; "
                                      "end_ms: zfv_snapshot_end after them (wait + reorder on the host)"},
-        "host_buffer_rate": {"ms_per_call": rate_ms, "cell_stage_per_s": n / (rate_ms * 1e-3),
+        "host_buffer_rate": {"ms_per_call": rate_ms, "cell_stage_per_s": n_interior / (rate_ms * 1e-3),
                              "note": "zfv_rate on host buffers (one RoC evaluation incl. PCIe and layout "
                                      "conversion), the drop-in RateOfChange::compute path"},
         "tuning": s.tune(-1).as_dict(),
@@ -347,7 +379,7 @@ def main():
         out["serving"] = serving(z, rates, args.serve_requests, args.serve_budget_gib)
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget, n_interior)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -44,6 +44,8 @@ def load():
         "zfv_ref_step": (c_int, [vp, dp, c_double, c_double]),
         "zfv_ref_cfl_dt": (c_int, [vp, dp, c_double, dp]),
         "zfv_ref_set_steady_state": (c_int, [vp, dp]),
+        "zfv_ref_advance": (c_int, [vp, dp, c_double, c_double, c_double, c_int, c_double, c_int, dp,
+                                    ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
         "zfv_ref_set_threads": (c_int, [vp, c_int]),
         "zfv_ref_eval": (c_int, [vp, c_int, dp, dp]),
         "zfv_ref_geometry": (c_int, [vp, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]),
@@ -121,6 +123,16 @@ class Oracle:
         dt = ctypes.c_double()
         _check(self.lib.zfv_ref_cfl_dt(self.h, _dp(u), cfl, ctypes.byref(dt)))
         return dt.value
+
+    def advance(self, u, cfl, t0, t_end, max_steps=0, drho_crit_rel=0.0, max_exponent=0):
+        """TimeLoop with LocalCFL (and RejectLargeDensityChange when
+        drho_crit_rel > 0); returns (u, t_final, steps, rejected)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).copy()
+        t = ctypes.c_double()
+        k, r = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.zfv_ref_advance(self.h, _dp(u), cfl, t0, t_end, max_steps, drho_crit_rel, max_exponent,
+                                        ctypes.byref(t), ctypes.byref(k), ctypes.byref(r)))
+        return u, t.value, k.value, r.value
 
     def set_steady_state(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -95,6 +95,15 @@ int zfv_ref_cfl_dt(void *h, const double *u, double cfl, double *dt) {
   })
 }
 
+int zfv_ref_advance(void *h, double *u, double cfl, double t0, double t_end, int max_steps, double drho_crit_rel,
+                    int max_exponent, double *t_final, int *steps, int *rejected) {
+  GUARD({
+    *t_final = static_cast<Solver *>(h)->advance(u, cfl, t0, t_end, max_steps, drho_crit_rel, max_exponent, steps,
+                                                 rejected);
+    return 0;
+  })
+}
+
 int zfv_ref_set_steady_state(void *h, const double *u) {
   static_cast<Solver *>(h)->set_steady_state(u);
   return 0;

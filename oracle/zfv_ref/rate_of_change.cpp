@@ -438,7 +438,11 @@ Solver::Solver(std::shared_ptr<Grid> grid, const Params &params,
 
   // make_reconstruction_array (include/zisa/reconstruction/global_reconstruction_decl.hpp:105-147)
   impl_->rc.resize(g.n_cells);
+  // cells are independent: built on the host's threads, identical to a serial loop
+  std::string setup_err;
+#pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < g.n_cells; ++i) {
+   try {
     LocalRC &r = impl_->rc[i];
     r.i = i;
     r.fam = &families_[i];
@@ -462,7 +466,12 @@ Solver::Solver(std::shared_ptr<Grid> grid, const Params &params,
     r.eq.eos = &impl_->eos;
     r.eq.gravity = &impl_->gravity;
     r.rhoEbar.assign(r.fam->l2g.size(), RhoE{});
+   } catch (const std::exception &e) {
+#pragma omp critical
+    if (setup_err.empty()) setup_err = "cell " + std::to_string(i) + ": " + e.what();
+   }
   }
+  if (!setup_err.empty()) throw Error(setup_err);
 
   // FluxLoop constructor: interior/exterior split (include/zisa/fvm_loops/flux_loop.hpp:29-90)
   for (int i = 0; i < g.n_cells; ++i) {
@@ -863,6 +872,62 @@ double Solver::cfl_dt(const double *u, double cfl) const {
     m = std::min(m, grid_->inradii[i] / ev);
   }
   return cfl * m;
+}
+
+// TimeLoop::operator() (src/zisa/fvm_loops/time_loop.cpp:36-75): a step of
+// the picked dt, the step-rejection check, on rejection reject_step
+// (:182-196: retry from u0 with the factor times the rejected dt), else the
+// clock advances and the next dt is picked from the new state
+// (pick_time_step, :77-81: factor * LocalCFL).  RejectLargeDensityChange
+// (src/zisa/ode/step_rejection.cpp:17-57): a step is bad when any cell's
+// density changes by more than drho_crit_rel times its old value; the factor
+// halves after a bad step, doubles after two good steps in a row, and stays
+// within [2^-max_exponent, 1].  The simulation clock (its library is not in
+// the reference's tree) clamps every step to the end time; a step that no
+// longer advances the time, or is not positive, ends the loop.
+double Solver::advance(double *u, double cfl, double t0, double t_end, int max_steps, double drho_crit_rel,
+                       int max_exponent, int *steps, int *rejected) {
+  const size_t n = size_t(grid_->n_cells) * 5;
+  const bool reject = drho_crit_rel > 0.0;
+  const double min_factor = reject ? std::pow(2.0, -double(max_exponent)) : 1.0;
+  double factor = 1.0;
+  bool was_good = true;
+  double t = t0;
+  int k = 0, r = 0;
+  if (max_steps <= 0) max_steps = 0x7fffffff;
+  auto clamp = [&](double dt) { return dt > t_end - t ? t_end - t : dt; };
+  std::vector<double> u0(n);
+  double dt = clamp(factor * cfl_dt(u, cfl));
+  while (t < t_end && k < max_steps) {
+    if (!(dt > 0.0) || t + dt == t) break;
+    std::copy(u, u + n, u0.begin());
+    step(u, t, dt);
+    if (reject) {
+      bool good = true;
+      for (int i = 0; i < grid_->n_cells; ++i)
+        if (std::abs(u0[5 * size_t(i)] - u[5 * size_t(i)]) > drho_crit_rel * u0[5 * size_t(i)]) good = false;
+      if (!good) {
+        was_good = false;
+        factor /= 2.0;
+      } else {
+        if (was_good) factor *= 2.0;
+        was_good = true;
+      }
+      factor = std::max(min_factor, std::min(factor, 1.0));
+      if (!good) {
+        ++r;
+        dt = clamp(factor * dt);
+        std::copy(u0.begin(), u0.end(), u);
+        continue;
+      }
+    }
+    t += dt;
+    ++k;
+    dt = clamp(factor * cfl_dt(u, cfl));
+  }
+  if (steps) *steps = k;
+  if (rejected) *rejected = r;
+  return t;
 }
 
 }  // namespace zfv_ref

@@ -498,19 +498,29 @@ StencilFamily make_first_order_family(int i) {
 }
 
 // src/zisa/reconstruction/stencil_family.cpp:96-119
+// (cells are independent: the loop runs on the host's threads; every cell's
+// family is the same as in a serial loop)
 std::vector<StencilFamily> compute_stencil_families(const Grid &grid, const Params &params) {
   std::vector<StencilFamily> out(grid.n_cells);
+  std::string err;
+#pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < grid.n_cells; ++i) {
-    const auto &fl = grid.cell_flags[i];
-    if (fl.interior || fl.ghost_cell_l1) {
-      out[i] = make_family(grid, i, params.orders, params.biases, params.overfit, params.n_stencils);
-    } else {
-      // StencilFamilyParams{{1}, {"c"}, {1.0}}: a one-cell central stencil
-      int o[1] = {1}, b[1] = {0};
-      double f[1] = {1.0};
-      out[i] = make_family(grid, i, o, b, f, 1);
+    try {
+      const auto &fl = grid.cell_flags[i];
+      if (fl.interior || fl.ghost_cell_l1) {
+        out[i] = make_family(grid, i, params.orders, params.biases, params.overfit, params.n_stencils);
+      } else {
+        // StencilFamilyParams{{1}, {"c"}, {1.0}}: a one-cell central stencil
+        int o[1] = {1}, b[1] = {0};
+        double f[1] = {1.0};
+        out[i] = make_family(grid, i, o, b, f, 1);
+      }
+    } catch (const std::exception &e) {
+#pragma omp critical
+      if (err.empty()) err = "cell " + std::to_string(i) + ": " + e.what();
     }
   }
+  if (!err.empty()) throw Error(err);
   return out;
 }
 

@@ -253,6 +253,11 @@ class Solver {
   // one SSP3 step in Butcher form, frozen BC after every stage sum.
   void step(double *u, double t, double dt);
   double cfl_dt(const double *u, double cfl) const;
+  // TimeLoop::operator() with LocalCFL and RejectNothing (drho_crit_rel <= 0)
+  // or RejectLargeDensityChange(drho_crit_rel, max_exponent); every step
+  // clamped to t_end; advances u in place, returns the final time
+  double advance(double *u, double cfl, double t0, double t_end, int max_steps, double drho_crit_rel,
+                 int max_exponent, int *steps, int *rejected);
   void set_steady_state(const double *u);
   void set_threads(int n);
 

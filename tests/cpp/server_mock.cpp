@@ -194,10 +194,11 @@ int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *
 int main(int argc, char **argv) {
   const int n_req = argc > 1 ? std::atoi(argv[1]) : 48;
   const double rate = argc > 2 ? std::atof(argv[2]) : 400.0;  // requests per second
+  const int max_members = argc > 3 ? std::atoi(argv[3]) : 64;  // slots per batch
   std::mt19937_64 rng(7);
   std::uniform_real_distribution<double> U01(0.0, 1.0);
   zfv_server *S = nullptr;
-  if (zfv_server_create(int64_t(1) << 30, int64_t(1) << 23, 64, 2, &S) != ZFV_OK) fail("create");
+  if (zfv_server_create(int64_t(1) << 30, int64_t(1) << 23, max_members, 2, &S) != ZFV_OK) fail("create");
   struct Req {
     zfv_mesh mesh;
     std::vector<int32_t> vid;

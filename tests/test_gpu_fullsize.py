@@ -1,0 +1,133 @@
+"""Parity at the benchmark configurations' full sizes and over the reference's
+step counts: the HIP rate of change against the oracle per variable on the
+whole 2D well-balanced mesh (263,520 triangles) and the whole 3D polytrope
+ball (~2M tetrahedra); the 2D atmosphere held at rest to its end time; the
+device time loop and batch members against the oracle's TimeLoop."""
+import os
+
+import numpy as np
+import pytest
+
+import zisafvm_amd as z
+from zisafvm_amd import workloads as W
+from oracle.zfv_ref import Oracle
+from conftest import rel_err, wb_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except Exception:
+        return 8
+
+
+def _solver(wl):
+    s = z.Solver(wl.mesh, wl.params)
+    s.set_state(wl.u0)
+    if wl.steady is not None:
+        s.set_steady_state(wl.steady)
+    return s
+
+
+def test_isothermal2d_full_size_rate_matches_oracle(gpu):
+    # 2D well-balanced configuration: n = 360 lattice, CWENO-AO order 4, HLLC,
+    # 1e-6 Gaussian pressure bump
+    wl = W.isothermal_2d(n=360, amplitude=1e-6)
+    print("isothermal2d cells", wl.mesh.n_cells, flush=True)
+    Lg = _solver(wl).rate(wl.u0)
+    Lo = Oracle(wl.mesh, wl.params, threads=_threads()).rate(wl.u0)
+    errs = wb_err(Lg, Lo, wl.mesh, wl.u0, wl.params.gamma)
+    print("isothermal2d n=360 wb err per variable", errs, flush=True)
+    assert max(errs) <= TOL
+
+
+def test_polytrope3d_full_size_rate_matches_oracle(gpu):
+    # 3D well-balanced configuration: n = 86 ball (~2M tetrahedra), n = 1
+    # polytrope, CWENO-AO order 3, Rusanov, 1e-5 radial-velocity bump
+    wl = W.polytrope_3d(n=86, amplitude=1e-5)
+    print("polytrope3d cells", wl.mesh.n_cells, flush=True)
+    Lg = _solver(wl).rate(wl.u0)
+    Lo = Oracle(wl.mesh, wl.params, threads=_threads()).rate(wl.u0)
+    errs = wb_err(Lg, Lo, wl.mesh, wl.u0, wl.params.gamma)
+    print("polytrope3d n=86 wb err per variable", errs, flush=True)
+    assert max(errs) <= TOL
+
+
+def test_isothermal2d_equilibrium_held_to_end_time(gpu):
+    # the discrete isothermal atmosphere (no bump) through the configuration's
+    # whole run, t_end = 0.25 at CFL 0.45 (about 1,500 SSP3 steps), on the
+    # device time loop: every cell stays at rest to round-off
+    wl = W.isothermal_2d(n=360, amplitude=0.0)
+    s = _solver(wl)
+    t, k = s.advance(wl.cfl, 0.0, 0.25, check_every=64)
+    u = s.get_state()
+    ng = wl.mesh.ghost == 0
+    drift = [np.abs(u[ng, v] - wl.u0[ng, v]).max() / np.abs(wl.u0[ng, v]).max() for v in (0, 4)]
+    mom = np.abs(u[ng, 1:3]).max() / np.abs(wl.u0[ng, 4]).max()
+    print("isothermal2d equilibrium: steps", k, "t", t, "drift rho,E", drift, "momentum", mom, flush=True)
+    assert t == 0.25 and k > 1000
+    assert max(drift) <= 1e-12 and mom <= 1e-12
+
+
+@pytest.mark.parametrize(
+    "name,make,t_end,reject",
+    [
+        ("vortex2d", lambda: W.vortex_2d(n=32), 0.02, False),
+        ("isothermal2d", lambda: W.isothermal_2d(n=24, amplitude=1e-3, order=3), 0.01, True),
+        ("isentropic2d", lambda: W.isentropic_2d(n=16, amplitude=1e-3, flux_bc=2), 0.01, False),
+    ],
+)
+def test_device_time_loop_matches_oracle_time_loop(gpu, name, make, t_end, reject):
+    # zfv_advance against the oracle's TimeLoop (LocalCFL each step, clamped
+    # to the end time, RejectLargeDensityChange when asked): the same number
+    # of accepted and rejected steps, the same end time, states within
+    # round-off accumulated over the run
+    wl = make()
+    o = Oracle(wl.mesh, wl.params)
+    if wl.steady is not None:
+        o.set_steady_state(wl.steady)
+    crit, mexp = 0.0, 0
+    if reject:
+        dt = o.cfl_dt(wl.u0, wl.cfl)
+        crit = 0.7 * float(np.max(np.abs(o.step(wl.u0, 0.0, dt)[:, 0] - wl.u0[:, 0]) / wl.u0[:, 0]))
+        mexp = 3
+    uo, to, ko, ro = o.advance(wl.u0, wl.cfl, 0.0, t_end, drho_crit_rel=crit, max_exponent=mexp)
+    s = _solver(wl)
+    t, k = s.advance(wl.cfl, 0.0, t_end, drho_crit_rel=crit, max_exponent=mexp)
+    ug = s.get_state()
+    errs = rel_err(ug - wl.u0, uo - wl.u0, mask=wl.mesh.ghost == 0)
+    print(name, "steps", k, ko, "rejected", s.rejected_steps, ro, "t", t, to, "err of u - u0", errs, flush=True)
+    assert k == ko and s.rejected_steps == ro and abs(t - to) <= 1e-14 * to
+    if reject:
+        assert ro > 0
+    assert max(errs) <= 1e-10
+
+
+def test_batch_members_match_oracle_time_loops(gpu):
+    # members of one batch, each advanced by its own CFL steps inside the
+    # batch, against the oracle's TimeLoop of that member alone
+    wls = [W.vortex_2d(n=12, beta=4.0), W.isothermal_2d(n=10, amplitude=1e-3, order=3), W.vortex_2d(n=20, beta=2.0)]
+    t_ends = [0.01, 0.02, 0.015]
+    cap = sum(w.mesh.n_cells for w in wls)
+    b = z.Batch(wls[0].params, cell_capacity=cap, face_capacity=2 * cap, max_instances=4)
+    solvers, slots = [], []
+    for w, te in zip(wls, t_ends):
+        s = _solver(w)
+        solvers.append(s)
+        slots.append(b.add(s, w.cfl, 0.0, te, 50))
+    b.run(60)
+    for w, te, s, slot in zip(wls, t_ends, solvers, slots):
+        o = Oracle(w.mesh, w.params)
+        if w.steady is not None:
+            o.set_steady_state(w.steady)
+        uo, to, ko, _ = o.advance(w.u0, w.cfl, 0.0, te, max_steps=50)
+        q = b.query(slot)
+        errs = rel_err(s.get_state() - w.u0, uo - w.u0, mask=w.mesh.ghost == 0)
+        print(w.name, "batch steps", q["steps"], "oracle", ko, "t", q["t"], to, "err", errs, flush=True)
+        assert q["steps"] == ko and abs(q["t"] - to) <= 1e-14 * to
+        assert max(errs) <= 1e-10
+    b.close()

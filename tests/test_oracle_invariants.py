@@ -334,3 +334,26 @@ def test_vortex_convergence_fixture():
     assert 2.8 <= c3["observed_orders"][-1] <= 3.35
     c4 = next(c for c in fx["cases"] if c["order"] == 4)
     assert c4["observed_orders"][-1] >= 3.5
+
+
+def test_time_loop_reaches_the_end_time_and_rejects_as_stated():
+    # TimeLoop (src/zisa/fvm_loops/time_loop.cpp:36-75) on the oracle: the
+    # last step is clamped so that the end time is reached exactly; a density
+    # criterion no step violates changes nothing (bit for bit); one that every
+    # full CFL step violates rejects steps, and the accepted ones still end at
+    # the end time with the state positive
+    wl = W.vortex_2d(n=12)
+    o = Oracle(wl.mesh, wl.params)
+    u1, t1, k1, r1 = o.advance(wl.u0, wl.cfl, 0.0, 0.01)
+    assert t1 == 0.01 and r1 == 0 and k1 > 1
+    u2, t2, k2, r2 = o.advance(wl.u0, wl.cfl, 0.0, 0.01, drho_crit_rel=1e9, max_exponent=3)
+    assert (t2, k2, r2) == (t1, k1, 0)
+    np.testing.assert_array_equal(u2, u1)
+    # the change of density over one full CFL step
+    dt = o.cfl_dt(wl.u0, wl.cfl)
+    ch = np.max(np.abs(o.step(wl.u0, 0.0, dt)[:, 0] - wl.u0[:, 0]) / wl.u0[:, 0])
+    u3, t3, k3, r3 = o.advance(wl.u0, wl.cfl, 0.0, 0.01, drho_crit_rel=0.7 * ch, max_exponent=3)
+    assert r3 > 0 and t3 == 0.01 and k3 > k1 and np.all(u3[:, 0] > 0)
+    # max_steps stops the loop early
+    _, t4, k4, _ = o.advance(wl.u0, wl.cfl, 0.0, 0.01, max_steps=2)
+    assert k4 == 2 and t4 < 0.01

@@ -154,7 +154,11 @@ void gpu_loop(zfv_server *S) {
       joining.assign(S->ready.begin(), S->ready.end());
       S->ready.clear();
     }
-    // join at the step boundary
+    // join at the step boundary; a request whose batch has no free slot or
+    // cells right now waits for the next boundary while other members of
+    // that batch are still running (they free slots as they retire), and
+    // fails only when it could not join even an empty batch
+    std::vector<std::shared_ptr<Request>> deferred;
     for (auto &r : joining) {
       Shape s = shape_for(r->params);
       ShapeKey k = key_of(s);
@@ -174,6 +178,14 @@ void gpu_loop(zfv_server *S) {
       }
       int32_t slot = -1;
       if (rc == ZFV_OK) rc = zfv_batch_add(b, r->h, r->cfl, 0.0, r->t_end, r->max_steps, &slot);
+      if (rc == ZFV_ERR_MEMORY && b) {
+        bool others = false;
+        for (const auto &q : S->running) others = others || q->batch == b;
+        if (others) {
+          deferred.push_back(r);
+          continue;
+        }
+      }
       std::lock_guard<std::mutex> lk(S->mu);
       if (rc != ZFV_OK) {
         r->state = ZFV_REQ_FAILED;
@@ -191,6 +203,10 @@ void gpu_loop(zfv_server *S) {
       r->state = ZFV_REQ_RUNNING;
       r->t_start = clock_t_::now();
       S->running.push_back(r);
+    }
+    if (!deferred.empty()) {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->ready.insert(S->ready.begin(), deferred.begin(), deferred.end());
     }
     // one step of every batch with active members
     std::vector<zfv_batch *> bs;

@@ -771,6 +771,8 @@ void pseudo_inverse(const std::vector<double> &A, int rows, int cols, std::vecto
   P = b;
 }
 
+}  // namespace
+
 // ---- Hilbert order -------------------------------------------------------------
 uint64_t hilbert2(uint32_t x, uint32_t y, int bits) {
   uint64_t d = 0;
@@ -814,7 +816,6 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
   return d;
 }
 
-}  // namespace
 
 Shape shape_for(const zfv_params &p) {
   Shape s;

@@ -74,6 +74,12 @@ struct HostImage {
   double linear_weights[ZFV_MAX_STENCILS] = {0};
 };
 
+// Hilbert index of integer coordinates with `bits` bits per axis (bits <= 32
+// in 2D, <= 21 in 3D): the cell order of the instances and of the mesh
+// renumbering tool.
+uint64_t hilbert2(uint32_t x, uint32_t y, int bits);
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits);
+
 // Builds everything on the host.  Throws ZfvError on unsupported input.
 void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img);
 
