@@ -163,6 +163,24 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
                 double drho_crit_rel, int32_t max_exponent, double *t_final, int32_t *steps_done,
                 int32_t *rejected);
 
+/* The device time loop of zfv_advance (same arguments, same steps, same
+ * results) for callers that cannot allocate: code that must not call
+ * hipMalloc or instantiate graphs (inside a capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  It
+ * allocates nothing: the loop state and, with step rejection, the backup of
+ * the state live in a file-scope device array of 16 MiB that each GPU the
+ * library runs on holds from module load until the process ends; steps are
+ * launched one by one (no graph) with the handle's tuned launch configuration
+ * (one wavefront per block if it was never tuned).  Limit: with step
+ * rejection (drho_crit_rel > 0) the state must fit 16 MiB - 256 B
+ * (nvars * n_cells * 8 bytes: 524,280 triangles or 419,424 tetrahedra),
+ * else ZFV_ERR_MEMORY.  The handle's device must be current.  Calls on one
+ * GPU share the array: calls that overlap on one GPU (from several host
+ * threads) race and must not be issued. */
+int zfv_advance_noalloc(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps,
+                        int32_t check_every, double drho_crit_rel, int32_t max_exponent, double *t_final,
+                        int32_t *steps_done, int32_t *rejected);
+
 /* LocalCFL::operator() on the device-resident state
  * (include/zisa/model/local_cfl_condition_impl.hpp:25-40). */
 int zfv_cfl_dt(zfv_handle *h, double cfl_number, double *dt);

@@ -204,6 +204,8 @@ def load_library(path: str = LIB_PATH):
         "zfv_cfl_dt": (c_int, [vp, c_double, dp]),
         "zfv_advance": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, c_double,
                                 ctypes.c_int32, dp, ip, ip]),
+        "zfv_advance_noalloc": (c_int, [vp, c_double, c_double, c_double, ctypes.c_int32, ctypes.c_int32, c_double,
+                                        ctypes.c_int32, dp, ip, ip]),
         "zfv_synchronize": (c_int, [vp]),
         "zfv_snapshot_begin": (c_int, [vp]),
         "zfv_snapshot_end": (c_int, [vp, dp]),
@@ -400,6 +402,19 @@ class Solver:
         r = ctypes.c_int32(0)
         _check(self.lib.zfv_advance(self.h, cfl, t0, t_end, max_steps, check_every, drho_crit_rel, max_exponent,
                                     ctypes.byref(t), ctypes.byref(k), ctypes.byref(r)))
+        self.rejected_steps = r.value
+        return t.value, k.value
+
+    def advance_noalloc(self, cfl: float, t0: float, t_end: float, max_steps: int = 0, check_every: int = 16,
+                        drho_crit_rel: float = 0.0, max_exponent: int = 0):
+        """The same time loop as advance() through zfv_advance_noalloc, which
+        allocates nothing (its loop state lives in the library's device
+        scratch array); returns (t_final, accepted steps)."""
+        t = ctypes.c_double(0.0)
+        k = ctypes.c_int32(0)
+        r = ctypes.c_int32(0)
+        _check(self.lib.zfv_advance_noalloc(self.h, cfl, t0, t_end, max_steps, check_every, drho_crit_rel,
+                                            max_exponent, ctypes.byref(t), ctypes.byref(k), ctypes.byref(r)))
         self.rejected_steps = r.value
         return t.value, k.value
 

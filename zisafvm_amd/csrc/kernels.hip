@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -814,31 +815,83 @@ static void drop_graphs(zfv_handle *h) {
 }
 
 // one step with the dt the device chooses (LocalCFL, clamped to t_end);
-// with reject, the step is undone when the density changes too much
-static void enqueue_adaptive_step(zfv_handle *h, bool reject) {
+// with reject, the step is undone when the density changes too much.  adv is
+// the loop state and Ubk the backup of u0 (used only with reject).
+static void enqueue_adaptive_step(zfv_handle *h, bool reject, int cfg, AdvState *adv, double *Ubk) {
   const Img &m = h->dimg;
   const int64_t count = int64_t(h->img.s.nvars) * m.n;
   if (reject)
-    hipLaunchKernelGGL(k_adv_backup, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->adv, count, h->U,
-                       h->Ubk);
-  h->L.cfl(m, h->U, &h->adv->dt_bits, h->stream);
-  hipLaunchKernelGGL(k_adv_choose, dim3(1), dim3(1), 0, h->stream, h->adv);
+    hipLaunchKernelGGL(k_adv_backup, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, adv, count, h->U, Ubk);
+  h->L.cfl(m, h->U, &adv->dt_bits, h->stream);
+  hipLaunchKernelGGL(k_adv_choose, dim3(1), dim3(1), 0, h->stream, adv);
   const double *Us = h->U;
   for (int s = 0; s < 3; ++s) {
-    h->L.rec[h->ph](m, Us, h->fpv, h->src, h->rec_cfg, frozen(h), h->stream);
+    h->L.rec[h->ph](m, Us, h->fpv, h->src, cfg, frozen(h), h->stream);
     h->L.face(m, h->fpv, h->G, h->stream);
     double *Un = (s == 2) ? h->U : h->X;
-    h->L.update(m, ssp3_coeffs(s), s, 0.0, &h->adv->dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
+    h->L.update(m, ssp3_coeffs(s), s, 0.0, &adv->dt, Us, h->G, h->src, h->U, h->K, Un, h->stream);
     Us = h->X;
   }
   if (reject) {
     // density is the first SoA plane of U
-    hipLaunchKernelGGL(k_adv_check, dim3(blocks_for(m.n, 256)), dim3(256), 0, h->stream, h->adv, m.n, h->U,
-                       h->Ubk);
-    hipLaunchKernelGGL(k_adv_restore, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->adv, count, h->U,
-                       h->Ubk);
+    hipLaunchKernelGGL(k_adv_check, dim3(blocks_for(m.n, 256)), dim3(256), 0, h->stream, adv, m.n, h->U, Ubk);
+    hipLaunchKernelGGL(k_adv_restore, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, adv, count, h->U, Ubk);
   }
-  hipLaunchKernelGGL(k_adv_end, dim3(1), dim3(1), 0, h->stream, h->adv);
+  hipLaunchKernelGGL(k_adv_end, dim3(1), dim3(1), 0, h->stream, adv);
+}
+
+// The loop state of a device time loop starting at t0 (zfv_advance and its
+// allocation-free variant fill it the same way)
+static AdvState adv_start(double cfl_number, double t0, double t_end, int32_t max_steps, double drho_crit_rel,
+                          int32_t max_exponent) {
+  AdvState a{};
+  a.t = t0;
+  a.t_end = t_end;
+  a.cfl = cfl_number;
+  a.steps = 0;
+  a.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
+  a.active = t0 < t_end ? 1 : 0;
+  a.drho_crit = drho_crit_rel > 0.0 ? drho_crit_rel : 0.0;
+  a.factor = 1.0;
+  a.min_factor = drho_crit_rel > 0.0 ? std::pow(2.0, -double(max_exponent)) : 1.0;
+  a.prev_good = 1;
+  a.dt_bits = adv_no_bound();
+  a.bad = 0;
+  return a;
+}
+
+static void check_advance_args(double cfl_number, double drho_crit_rel, int32_t max_exponent) {
+  if (!(cfl_number > 0.0)) throw ZfvError(ZFV_ERR_INVALID, "cfl_number must be > 0");
+  if (drho_crit_rel > 0.0 && max_exponent < 0) throw ZfvError(ZFV_ERR_INVALID, "max_exponent must be >= 0");
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free device time loop (zfv_advance_noalloc): the
+// loop state (one AdvState) in the first kNoAllocStateBytes, then the backup
+// of u0 that step rejection restores from.  A file-scope device array: each
+// GPU the library runs on holds its own copy from module load until the
+// process ends, and every call on that GPU shares it.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr size_t kNoAllocStateBytes = 256;
+static_assert(sizeof(AdvState) <= kNoAllocStateBytes, "loop state larger than its scratch slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev`, which must be current
+// (looked up once per device; the array's name is only a host-side stand-in)
+static unsigned char *scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw ZfvError(ZFV_ERR_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ZfvError(ZFV_ERR_HIP, "no device address for the time-loop scratch");
+    g_scratch_dev[dev] = p;
+  }
+  return static_cast<unsigned char *>(g_scratch_dev[dev]);
 }
 
 static void to_soa(const zfv_handle *h, const double *aos, double *soa) { zfv::to_soa(h->img, aos, soa); }
@@ -1110,9 +1163,8 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
                 double drho_crit_rel, int32_t max_exponent, double *t_final, int32_t *steps_done,
                 int32_t *rejected) {
   ZFV_API_BEGIN
-  if (!(cfl_number > 0.0)) throw ZfvError(ZFV_ERR_INVALID, "cfl_number must be > 0");
+  check_advance_args(cfl_number, drho_crit_rel, max_exponent);
   if (check_every < 1) check_every = 16;
-  if (drho_crit_rel > 0.0 && max_exponent < 0) throw ZfvError(ZFV_ERR_INVALID, "max_exponent must be >= 0");
   rec_config(h, h->U);
   if (!h->adv) h->adv = h->alloc<AdvState>(1);
   const bool reject = drho_crit_rel > 0.0;
@@ -1121,29 +1173,56 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
   if (!step) {
     hipGraph_t graph;
     HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_adaptive_step(h, reject);
+    enqueue_adaptive_step(h, reject, h->rec_cfg, h->adv, h->Ubk);
     HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
     HIP_CHECK(hipGraphInstantiate(&step, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
   }
-  AdvState a{};
-  a.t = t0;
-  a.t_end = t_end;
-  a.cfl = cfl_number;
-  a.steps = 0;
-  a.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
-  a.active = t0 < t_end ? 1 : 0;
-  a.drho_crit = drho_crit_rel > 0.0 ? drho_crit_rel : 0.0;
-  a.factor = 1.0;
-  a.min_factor = drho_crit_rel > 0.0 ? std::pow(2.0, -double(max_exponent)) : 1.0;
-  a.prev_good = 1;
-  a.dt_bits = adv_no_bound();
-  a.bad = 0;
+  AdvState a = adv_start(cfl_number, t0, t_end, max_steps, drho_crit_rel, max_exponent);
   HIP_CHECK(hipMemcpyAsync(h->adv, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   while (a.active) {
     for (int k = 0; k < check_every; ++k) HIP_CHECK(hipGraphLaunch(step, h->stream));
     HIP_CHECK(hipMemcpyAsync(&a, h->adv, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+  }
+  if (t_final) *t_final = a.t;
+  if (steps_done) *steps_done = a.steps;
+  if (rejected) *rejected = a.rejections;
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_advance_noalloc(zfv_handle *h, double cfl_number, double t0, double t_end, int32_t max_steps,
+                        int32_t check_every, double drho_crit_rel, int32_t max_exponent, double *t_final,
+                        int32_t *steps_done, int32_t *rejected) {
+  ZFV_API_BEGIN
+  check_advance_args(cfl_number, drho_crit_rel, max_exponent);
+  if (check_every < 1) check_every = 16;
+  int dev = -1;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev != h->dev) throw ZfvError(ZFV_ERR_INVALID, "zfv_advance_noalloc: make the handle's device current first");
+  const bool reject = drho_crit_rel > 0.0;
+  const size_t backup = sizeof(double) * size_t(h->img.s.nvars) * size_t(h->img.n);
+  if (reject && backup > kNoAllocScratchBytes - kNoAllocStateBytes)
+    throw ZfvError(ZFV_ERR_MEMORY, "zfv_advance_noalloc: step rejection backs up the state in " +
+                                       std::to_string((kNoAllocScratchBytes - kNoAllocStateBytes) >> 20) +
+                                       " MiB of scratch; this instance needs " + std::to_string(backup) +
+                                       " bytes (use zfv_advance)");
+  unsigned char *sp = scratch_on(dev);
+  AdvState *adv = reinterpret_cast<AdvState *>(sp);
+  double *Ubk = reinterpret_cast<double *>(sp + kNoAllocStateBytes);
+  // no tuning here (it creates events): the tuned launch configuration when
+  // there is one, else one wavefront per block; results do not depend on it
+  const int cfg = h->rec_cfg >= 0 ? h->rec_cfg : 0;
+  // the whole loop state is written before the first step reads any of it
+  AdvState a = adv_start(cfl_number, t0, t_end, max_steps, drho_crit_rel, max_exponent);
+  HIP_CHECK(hipMemcpyAsync(adv, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  while (a.active) {
+    for (int k = 0; k < check_every; ++k) enqueue_adaptive_step(h, reject, cfg, adv, Ubk);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(&a, adv, sizeof(a), hipMemcpyDeviceToHost, h->stream));
     HIP_CHECK(hipStreamSynchronize(h->stream));
   }
   if (t_final) *t_final = a.t;
