@@ -17,7 +17,8 @@ counted).  The JSON line also carries
     / its average HIP-event duration on the handle's stream;
   - cpu_baseline: the oracle (the reference's OpenMP CPU path restated) timed
     on all of this host's cores over a bounded sample, and on one thread.
-Opt-in legs:
+Opt-in legs (the library refuses batches above 65,536 cells or 8 members,
+so at the configurations' sizes both legs report that refusal):
   --batch  batched_vs_plain: the batched configuration (64 members, 5k-200k
            cells, order 3, per-member CFL every step, one graph per step for
            all members) against the same members stepped one after another on
@@ -149,6 +150,11 @@ def batched_vs_plain(z, steps):
 
     wls = W.batched_2d()
     cells = sum(w.mesh.n_cells for w in wls)
+    try:
+        b = z.Batch(wls[0].params, cell_capacity=cells, face_capacity=2 * cells, max_instances=len(wls))
+        b.close()
+    except z.ZfvError as e:
+        return {"members": len(wls), "cells": cells, "refused": str(e)}
     t0 = time.perf_counter()
     solvers = []
     for w in wls:
@@ -202,6 +208,11 @@ def serving(z, rates, n_requests, budget_gib):
     from zisafvm_amd import serving as S
     from zisafvm_amd import workloads as W
 
+    try:
+        b = z.Batch(z.make_params(), cell_capacity=1 << 23, face_capacity=1 << 24, max_instances=64)
+        b.close()
+    except z.ZfvError as e:
+        return {"requests": n_requests, "refused": str(e)}
     t0 = time.perf_counter()
     reqs = W.serving_requests(n_requests)
     t_gen = time.perf_counter() - t0

@@ -84,7 +84,14 @@ typedef struct zfv_params {
                                      physical flux), 2 EquilibriumFluxBC (the flux of the
                                      cell's isentropic equilibrium at the face points;
                                      with ZFV_EQ_ISENTROPIC only) */
-  int32_t steps_per_recompute;    /* must be 1: equilibria are re-solved every call */
+  int32_t steps_per_recompute;    /* >= 1; with k > 1 every cell keeps its equilibrium (theta
+                                     and scale) and recomputes it on every k-th rate
+                                     evaluation since the last recompute, or earlier when its
+                                     (rho, E_int) moves from the kept equilibrium's cell
+                                     average by recompute_threshold or more in the kept scale
+                                     (include/zisa/reconstruction/local_reconstruction.hpp:87-100);
+                                     autotuning and zfv_time_kernels launches recompute every
+                                     cell and leave the kept state alone */
   int32_t scaling;                /* 0 EulerScaling (the experiments' choice), 1 UnityScaling
                                      (include/zisa/model/characteristic_scale.hpp:14-46);
                                      the kernels implement EulerScaling */
@@ -240,11 +247,18 @@ int zfv_tune(zfv_handle *h, int32_t rec_threads, zfv_tuning *out);
  * (src/zisa/fvm_loops/time_loop.cpp:36-75): per step, dt = cfl * LocalCFL
  * clamped to t_end, one SSP3 step with the frozen BC.  All members of a batch
  * advance in one captured HIP graph per step (block tables map launched
- * blocks to members; no per-member launches).  A member's handle must stay
- * alive while it is in the batch; read its state with zfv_get_state after
- * zfv_batch_run returns. */
+ * blocks to members; no per-member launches).  An instance is a member of
+ * at most one batch; zfv_destroy of a member removes it from its batch first,
+ * zfv_set_steady_state on a member reaches the batch, and a snapshot
+ * (zfv_snapshot_begin) of a member is ordered before the batch's later
+ * steps.  Read a member's state with zfv_get_state after zfv_batch_run
+ * returns. */
 typedef struct zfv_batch zfv_batch;
-/* Capacity: total cells / interior faces / members the batch can hold. */
+/* Capacity: total cells / interior faces / members the batch can hold.
+ * Batches above 65,536 cells or 8 members are refused (ZFV_ERR_INVALID):
+ * those are the largest run on the GPU with checked results, and a batched
+ * run of 64 members and 4.0M cells ended in a GPU memory fault whose cause
+ * has not been found. */
 int zfv_batch_create(const zfv_params *shape_params, int64_t cell_capacity, int64_t face_capacity,
                      int32_t max_instances, zfv_batch **out);
 int zfv_batch_destroy(zfv_batch *b);
@@ -301,6 +315,8 @@ typedef struct zfv_server_stats_t {
   int64_t cell_stages;  /* cell-stage updates completed so far */
   int64_t steps_run;    /* step boundaries passed by the GPU loop */
 } zfv_server_stats_t;
+/* batch_cell_capacity and max_members size the server's batches (one per
+ * kernel shape, created on first use): within zfv_batch_create's limits. */
 int zfv_server_create(int64_t hbm_budget_bytes, int64_t batch_cell_capacity, int32_t max_members,
                       int32_t setup_workers, zfv_server **out);
 /* Copies the request; returns at once with its id. */

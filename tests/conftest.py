@@ -98,3 +98,27 @@ def shape_cases():
         "3d_order3_face_deg3": (b3, lambda: p3(edge_deg=3)),
         "3d_order3_three_one_sided": (b3, lambda: p3(n_stencils=4)),
     }
+
+
+def with_recompute(params, steps, threshold):
+    """A copy of params with steps_per_recompute and recompute_threshold set."""
+    q = type(params).from_buffer_copy(params)
+    q.steps_per_recompute = steps
+    q.recompute_threshold = threshold
+    return q
+
+
+def recompute_states(u, n_calls=7, rel=1e-4, seed=0):
+    """States for successive rate evaluations of kept-equilibrium tests: the
+    second repeats the first; each later one scales every cell's conserved
+    variables (velocity kept, rho and E_int scaled alike) by 1 + rel N(0, 1)
+    against the first, so that under a threshold of about rel some cells
+    recompute and others keep their equilibrium."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = [u, u]
+    for _ in range(n_calls - 2):
+        f = 1.0 + rel * rng.standard_normal(u.shape[0])
+        out.append(u * f[:, None])
+    return out

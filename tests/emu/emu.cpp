@@ -5,6 +5,7 @@
 // (zisafvm_amd/csrc/stage_math.hpp) in host loops, so the CPU test suite can
 // check the product's numerics against the oracle without a GPU.  The
 // product library never loads this; GPU tests call libzfv_hip.so.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -19,6 +20,7 @@
 #include "../../zisafvm_amd/csrc/common.hpp"
 #include "../../zisafvm_amd/csrc/setup.hpp"
 #include "../../zisafvm_amd/csrc/stage_math.hpp"
+#include "../../zisafvm_amd/csrc/batch.hpp"
 
 using namespace zfv;
 
@@ -28,7 +30,8 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim, pot, frm;
+  std::vector<double> U, fpv, G, src, phim, pot, frm, eqc;
+  std::vector<int32_t> eqcnt;
 };
 
 // The lanes of one cell's group run as NV host threads in lockstep: every
@@ -182,6 +185,15 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.K = p->polytrope_K;
     m.G = p->polytrope_G;
     m.flux_bc = p->flux_bc;
+    m.spr = p->steps_per_recompute;
+    m.eq_threshold = p->recompute_threshold;
+    if (p->steps_per_recompute > 1) {
+      // kept equilibria, as the handle keeps them on the device
+      e->eqc.assign(size_t(8) * im.n, 0.0);
+      e->eqcnt.assign(size_t(im.n), 0);
+      m.eqc = e->eqc.data();
+      m.eqcnt = e->eqcnt.data();
+    }
     for (int q = 0; q < 16; ++q)
       for (int d = 0; d < 3; ++d) m.shift[q][d] = im.shift[3 * q + d];
     const int fdim = im.s.dim == 2 ? 1 : 3;
@@ -354,6 +366,225 @@ int zfv_emu_poly_eval(int n_dims, int degree, const double *a, const double *c, 
   else if (n_dims == 3 && degree == 3) poly_eval<3, 3, 1>(aa, c, X, &px);
   else return -1;
   *out = px;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch emulation: the batch kernels' per-block functions (batch.hpp) run in
+// host loops over the block tables the library builds, with each member's
+// arrays on the host (U, X, K, face values, fluxes, source).  Every launch of
+// a step is emulated in the order of the captured step: begin, CFL, dt,
+// then per stage reconstruction, face fluxes, update; then the end of step.
+namespace {
+
+struct EmuMember {
+  Emu *e = nullptr;
+  std::vector<double> U, X, K, fpv, G, src, steady;
+};
+
+struct EmuBatch {
+  std::vector<EmuMember> mem;
+  std::vector<DevInst> inst;
+  Shape shape;
+  int ph = PH_LIGHT;
+  // counters of what the last run touched, for the tests
+  int64_t rec_groups = 0, face_threads = 0, cell_threads = 0;
+};
+
+template <int DIM, int DOFC, int RC, int NB, int RB, int Q>
+void batch_step(EmuBatch &B, int waves, bool xcd, int rec_sample) {
+  constexpr int NV = DIM + 2;
+  constexpr int GPW = groups_per_wave<NV>();
+  const int n_inst = int(B.inst.size());
+  std::vector<int64_t> cells(n_inst, 0), faces(n_inst, 0);
+  for (int s = 0; s < n_inst; ++s)
+    if (B.inst[s].active) {
+      cells[s] = B.inst[s].m.n;
+      faces[s] = B.inst[s].m.ne;
+    }
+  const int64_t big = int64_t(1) << 40;
+  const auto tc = block_table(cells, 256, big, xcd);
+  const auto tf = block_table(faces, 256, big, xcd);
+  const auto tr = block_table(cells, int64_t(waves) * GPW, big, xcd);
+  DevInst *insts = B.inst.data();
+  for (int s = 0; s < n_inst; ++s) kb_begin_instance(insts[s]);
+  // CFL: per block the minimum of its threads' candidates, then the minimum
+  // over blocks per instance (the kernel's atomic min on the bits of
+  // positive doubles)
+  for (const BlockRef &b : tc) {
+    if (b.inst < 0 || !insts[b.inst].active) continue;
+    double r = 1.0e300;
+    for (int t = 0; t < 256; ++t) r = std::min(r, kb_cfl_thread<DIM>(insts[b.inst], b, t));
+    unsigned long long bits;
+    std::memcpy(&bits, &r, sizeof(bits));
+    insts[b.inst].dt_bits = std::min(insts[b.inst].dt_bits, bits);
+  }
+  for (int s = 0; s < n_inst; ++s) kb_choose_instance(insts[s]);
+  for (int stage = 0; stage < 3; ++stage) {
+    // reconstruction: the NV lanes of a group run as host threads in lockstep
+    LaneBarrier bar(NV);
+    std::vector<std::thread> lanes;
+    std::vector<std::string> errs(NV);
+    std::atomic<int64_t> groups{0};
+    for (int g = 0; g < NV; ++g)
+      lanes.emplace_back([&, g] {
+        try {
+          EmuXch x{&bar, g};
+          for (size_t bi = 0; bi < tr.size(); ++bi) {
+            if (rec_sample > 1 && bi % size_t(rec_sample) != 0) continue;
+            for (int wave = 0; wave < waves; ++wave)
+              for (int grp = 0; grp < GPW; ++grp) {
+                if (B.ph == PH_LIGHT)
+                  kb_reconstruct_group<DIM, DOFC, RC, NB, RB, Q, PH_LIGHT>(insts, tr[bi], waves, wave, grp, g, stage, x);
+                else if (B.ph == PH_LIN)
+                  kb_reconstruct_group<DIM, DOFC, RC, NB, RB, Q, PH_LIN>(insts, tr[bi], waves, wave, grp, g, stage, x);
+                else
+                  kb_reconstruct_group<DIM, DOFC, RC, NB, RB, Q, PH_FULL>(insts, tr[bi], waves, wave, grp, g, stage, x);
+                if (g == 0) groups++;
+              }
+          }
+        } catch (const std::exception &ex) {
+          errs[g] = ex.what();
+        }
+      });
+    for (auto &t : lanes) t.join();
+    for (auto &m : errs)
+      if (!m.empty()) throw ZfvError(ZFV_ERR_INVALID, "batch emulation: " + m);
+    B.rec_groups += groups.load();
+    for (const BlockRef &b : tf)
+      for (int t = 0; t < 256; ++t) {
+        kb_face_flux_thread<DIM, Q>(insts, b, t);
+        ++B.face_threads;
+      }
+    for (const BlockRef &b : tc)
+      for (int t = 0; t < 256; ++t) {
+        kb_update_thread<DIM>(insts, b, t, stage);
+        ++B.cell_threads;
+      }
+  }
+  for (int s = 0; s < n_inst; ++s) kb_end_instance(insts[s]);
+}
+
+void batch_dispatch(EmuBatch &B, int waves, bool xcd, int rec_sample) {
+  const Shape &s = B.shape;
+#define EMU_BSHAPE(D, DOF, RC_, NB_, RB_, Q_) \
+  if (s.dim == D && s.dofc == DOF && s.rc == RC_ && s.nb == NB_ && s.rb == RB_ && s.q == Q_) \
+    return batch_step<D, DOF, RC_, NB_, RB_, Q_>(B, waves, xcd, rec_sample);
+  EMU_BSHAPE(2, 6, 10, 3, 3, 2)
+  EMU_BSHAPE(2, 6, 10, 3, 3, 3)
+  EMU_BSHAPE(2, 10, 18, 3, 3, 3)
+  EMU_BSHAPE(2, 10, 18, 3, 3, 2)
+  EMU_BSHAPE(2, 3, 4, 3, 3, 1)
+  EMU_BSHAPE(2, 3, 4, 3, 3, 2)
+  EMU_BSHAPE(3, 10, 18, 4, 4, 3)
+  EMU_BSHAPE(3, 10, 18, 4, 4, 4)
+  EMU_BSHAPE(3, 10, 18, 3, 4, 3)
+#undef EMU_BSHAPE
+  throw ZfvError(ZFV_ERR_INVALID, "batch emulation: shape not instantiated");
+}
+
+}  // namespace
+
+extern "C" {
+
+// A batch of emulated instances (all of one kernel shape), each with its
+// initial state (n x 5, caller order), optional steady state (frozen BC),
+// CFL number, end time and step limit.
+int zfv_emu_batch_create(void **members, int n, const double *const *u0, const double *const *steady,
+                         const double *cfl, const double *t_end, const int *max_steps, void **out) {
+  try {
+    auto B = std::make_unique<EmuBatch>();
+    B->mem.resize(n);
+    B->inst.resize(n);
+    int ph = -1;
+    for (int k = 0; k < n; ++k) {
+      Emu *e = static_cast<Emu *>(members[k]);
+      const HostImage &im = e->img;
+      const Shape &s = im.s;
+      if (k == 0) B->shape = s;
+      const Shape &b = B->shape;
+      if (s.dim != b.dim || s.dofc != b.dofc || s.rc != b.rc || s.nb != b.nb || s.rb != b.rb || s.q != b.q)
+        throw ZfvError(ZFV_ERR_INVALID, "members of different kernel shapes");
+      EmuMember &M = B->mem[k];
+      M.e = e;
+      const int64_t nn = im.n, ne = std::max<int64_t>(im.ne, 1);
+      const int nv = s.nvars;
+      M.U.assign(size_t(nv) * nn, 0.0);
+      M.X.assign(size_t(nv) * nn, 0.0);
+      M.K.assign(size_t(3) * nv * nn, 0.0);
+      M.fpv.assign(size_t(2) * s.q * nv * ne, 0.0);
+      M.G.assign(size_t(nv) * ne, 0.0);
+      if (e->m.gravity != 0) M.src.assign(size_t(nv) * nn, 0.0);
+      to_soa(im, u0[k], M.U.data());
+      DevInst d{};
+      d.m = e->m;
+      if (steady && steady[k]) {
+        M.steady.assign(size_t(nv) * nn, 0.0);
+        to_soa(im, steady[k], M.steady.data());
+        d.m.steady = M.steady.data();
+      }
+      d.U = M.U.data();
+      d.X = M.X.data();
+      d.K = M.K.data();
+      d.fpv = M.fpv.data();
+      d.G = M.G.data();
+      d.src = M.src.empty() ? nullptr : M.src.data();
+      d.cfl = cfl[k];
+      d.t = 0.0;
+      d.t_end = t_end[k];
+      d.active = 1;
+      d.max_steps = max_steps[k] > 0 ? max_steps[k] : 0x7fffffff;
+      B->inst[k] = d;
+      const int p = physics_set(d.m);
+      ph = ph < 0 ? p : physics_union(ph, p);
+    }
+    B->ph = ph < 0 ? PH_LIGHT : ph;
+    *out = B.release();
+    return 0;
+  } catch (const std::exception &ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+int zfv_emu_batch_destroy(void *b) {
+  delete static_cast<EmuBatch *>(b);
+  return 0;
+}
+
+// n_steps steps of the batch with reconstruction blocks of `waves`
+// wavefronts; rec_sample > 1 reconstructs only every rec_sample-th block of
+// the table (a bounds run at large sizes: the skipped blocks' face values
+// are stale, the results not meaningful)
+int zfv_emu_batch_run(void *bp, int n_steps, int waves, int xcd_chunks, int rec_sample) {
+  try {
+    EmuBatch &B = *static_cast<EmuBatch *>(bp);
+    if (waves != 1 && waves != 2 && waves != 4) throw ZfvError(ZFV_ERR_INVALID, "waves must be 1, 2 or 4");
+    for (int k = 0; k < n_steps; ++k) batch_dispatch(B, waves, xcd_chunks != 0, rec_sample);
+    return 0;
+  } catch (const std::exception &ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+int zfv_emu_batch_member(void *bp, int k, double *u_out, double *t, int *steps, int *active) {
+  EmuBatch &B = *static_cast<EmuBatch *>(bp);
+  if (k < 0 || k >= int(B.mem.size())) return -1;
+  from_soa(B.mem[k].e->img, B.mem[k].U.data(), u_out, false);
+  *t = B.inst[k].t;
+  *steps = B.inst[k].steps_done;
+  *active = B.inst[k].active;
+  return 0;
+}
+
+int zfv_emu_batch_counts(void *bp, int64_t *rec_groups, int64_t *face_threads, int64_t *cell_threads) {
+  EmuBatch &B = *static_cast<EmuBatch *>(bp);
+  *rec_groups = B.rec_groups;
+  *face_threads = B.face_threads;
+  *cell_threads = B.cell_threads;
   return 0;
 }
 

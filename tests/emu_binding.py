@@ -74,3 +74,57 @@ class Emu:
         n, ne, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
         self.lib.zfv_emu_info(self.h, ctypes.byref(n), ctypes.byref(ne), ctypes.byref(c))
         return n.value, ne.value, c.value
+
+
+class EmuBatch:
+    """The batch kernels' per-block functions over the library's block tables,
+    in host loops (zfv_emu_batch_* in tests/emu/emu.cpp)."""
+
+    def __init__(self, members, workloads, cfls, t_ends, max_steps):
+        self.lib = load()
+        lib = self.lib
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        lib.zfv_emu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(dp), ctypes.POINTER(dp),
+                                             dp, dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(vp)]
+        lib.zfv_emu_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.zfv_emu_batch_member.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]
+        lib.zfv_emu_batch_counts.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int64)] * 3
+        lib.zfv_emu_batch_destroy.argtypes = [vp]
+        n = len(members)
+        self.members, self.n_cells = members, [w.mesh.n_cells for w in workloads]
+        self._u0 = [np.ascontiguousarray(w.u0, dtype=np.float64) for w in workloads]
+        self._st = [None if w.steady is None else np.ascontiguousarray(w.steady, dtype=np.float64) for w in workloads]
+        hs = (vp * n)(*[m.h for m in members])
+        u0p = (dp * n)(*[u.ctypes.data_as(dp) for u in self._u0])
+        stp = (dp * n)(*[None if s is None else s.ctypes.data_as(dp) for s in self._st])
+        c = np.ascontiguousarray(cfls, dtype=np.float64)
+        te = np.ascontiguousarray(t_ends, dtype=np.float64)
+        ms = (ctypes.c_int * n)(*max_steps)
+        h = vp()
+        if lib.zfv_emu_batch_create(hs, n, u0p, stp, c.ctypes.data_as(dp), te.ctypes.data_as(dp), ms,
+                                    ctypes.byref(h)) != 0:
+            raise RuntimeError(lib.zfv_emu_last_error().decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.zfv_emu_batch_destroy(self.h)
+
+    def run(self, n_steps, waves=1, xcd_chunks=True, rec_sample=1):
+        if self.lib.zfv_emu_batch_run(self.h, n_steps, waves, int(xcd_chunks), rec_sample) != 0:
+            raise RuntimeError(self.lib.zfv_emu_last_error().decode())
+
+    def member(self, k):
+        u = np.zeros((self.n_cells[k], 5))
+        t = ctypes.c_double()
+        steps, active = ctypes.c_int(), ctypes.c_int()
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.lib.zfv_emu_batch_member(self.h, k, u.ctypes.data_as(dp), ctypes.byref(t), ctypes.byref(steps),
+                                      ctypes.byref(active))
+        return u, t.value, steps.value, bool(active.value)
+
+    def counts(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.lib.zfv_emu_batch_counts(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value

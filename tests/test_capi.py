@@ -51,14 +51,14 @@ def test_no_device_is_a_clean_error():
 
 def test_invalid_configuration_is_rejected_before_device_work():
     p = z.make_params(order=3)
-    p.steps_per_recompute = 2
+    p.steps_per_recompute = 0
     mesh = z.lattice_mesh_2d(6)
     try:
         z.Solver(mesh, p)
     except z.ZfvError as e:
         assert "steps_per_recompute" in str(e) or "device" in str(e)
     else:
-        raise AssertionError("accepted steps_per_recompute = 2")
+        raise AssertionError("accepted steps_per_recompute = 0")
 
 
 def test_lattice_mesh_counts_and_volumes():
@@ -126,7 +126,7 @@ def test_enumerated_fields_are_range_checked():
     mesh = z.lattice_mesh_2d(6)
     assert _estimate(mesh, z.make_params()) > 0
     bad = [("flux_bc", 3), ("flux_bc", -1), ("flux", 2), ("reconstruction", 2), ("equilibrium", 3), ("gravity", 4),
-           ("gamma", 1.0)]
+           ("gamma", 1.0), ("steps_per_recompute", 0)]
     for field, value in bad:
         p = z.make_params()
         setattr(p, field, value)
@@ -147,3 +147,31 @@ def test_enumerated_fields_are_range_checked():
     p = z.make_params(flux_bc=z.ZFV_FLUX_BC_ISENTROPIC, equilibrium=z.ZFV_EQ_ISENTROPIC, gravity=z.ZFV_GRAVITY_AXIAL,
                       gravity_g=1.0)
     assert _estimate(mesh, p) > 0
+    # kept equilibria with the isentropic flux BC (which solves the current
+    # equilibrium every call) are not implemented
+    p = z.make_params(flux_bc=z.ZFV_FLUX_BC_ISENTROPIC, equilibrium=z.ZFV_EQ_ISENTROPIC, gravity=z.ZFV_GRAVITY_AXIAL,
+                      gravity_g=1.0, steps_per_recompute=2)
+    try:
+        _estimate(mesh, p)
+    except z.ZfvError as e:
+        assert "steps_per_recompute" in str(e)
+    else:
+        raise AssertionError("accepted steps_per_recompute 2 with the isentropic flux BC")
+    # kept equilibria: 8 doubles and a counter per cell on top
+    assert _estimate(mesh, z.make_params(steps_per_recompute=3)) == _estimate(mesh, z.make_params()) + 68 * mesh.n_cells
+
+
+def test_batches_above_the_gpu_run_sizes_are_refused():
+    # zfv_batch_create checks its capacity before any device work
+    p = z.make_params()
+    for cells, members in ((65537, 2), (1024, 9), (4031374, 64)):
+        try:
+            z.Batch(p, cell_capacity=cells, face_capacity=2 * cells, max_instances=members)
+        except z.ZfvError as e:
+            assert "refused" in str(e), str(e)
+        else:
+            raise AssertionError(f"accepted a batch of {cells} cells, {members} members")
+    try:
+        z.Batch(p, cell_capacity=65536, face_capacity=131072, max_instances=8)
+    except z.ZfvError as e:
+        assert "refused" not in str(e) and "device" in str(e), str(e)

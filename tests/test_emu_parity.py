@@ -138,3 +138,26 @@ def test_isentropic_gamma2_periodic():
     o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)
     errs = wb_err(e.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, wl.params.gamma)
     assert max(errs) <= TOL, errs
+
+
+@pytest.mark.parametrize("name", ["isothermal2d", "polytrope3d"])
+def test_kept_equilibria_match_oracle(name):
+    # steps_per_recompute 3 with a threshold that about half the cells pass on
+    # each call: the kept theta, scale and counters of every cell follow the
+    # oracle's over seven successive rate evaluations
+    from conftest import recompute_states, with_recompute
+
+    if name == "isothermal2d":
+        wl = W.isothermal_2d(n=10, order=3, amplitude=1e-3, ghost_layers=0)
+    else:
+        wl = W.polytrope_3d(n=6, amplitude=1e-3)
+    p = with_recompute(wl.params, 3, 1e-4)
+    o, e = Oracle(wl.mesh, p), Emu(wl.mesh, p)
+    plain = Oracle(wl.mesh, wl.params)
+    kept = 0.0
+    for k, u in enumerate(recompute_states(wl.u0, n_calls=7, rel=1e-4, seed=5)):
+        Le, Lo = e.rate(u), o.rate(u)
+        errs = wb_err(Le, Lo, wl.mesh, u, p.gamma)
+        assert max(errs) <= TOL, (k, errs)
+        kept = max(kept, max(wb_err(Lo, plain.rate(u), wl.mesh, u, p.gamma)))
+    assert kept > 1e-11  # some calls kept equilibria of earlier states

@@ -73,3 +73,41 @@ def test_batch_admits_and_retires_between_steps(gpu):
     b.add(sd, 0.45, 0.0, 1.0, 1)
     with pytest.raises(z.ZfvError):
         b.add(z.Solver(a.mesh, a.params), 0.45, 0.0, 1.0, 1)
+
+
+def test_member_steady_state_snapshot_and_destroy_inside_a_batch(gpu):
+    # zfv_set_steady_state on a member reaches the batch (its frozen BC is
+    # on: the member still equals its own loop, which has the BC from the
+    # start); a snapshot begun before zfv_batch_run holds the state before
+    # the run; destroying a member removes it from the batch, and the others
+    # keep running
+    a, c = W.isothermal_2d(n=10, amplitude=1e-3, order=3), W.vortex_2d(n=12)
+    cap = a.mesh.n_cells + c.mesh.n_cells
+    b = z.Batch(a.params, cell_capacity=cap, face_capacity=2 * cap, max_instances=4)
+    sa, sc = z.Solver(a.mesh, a.params), z.Solver(c.mesh, c.params)
+    sa.set_state(a.u0)
+    sc.set_state(c.u0)
+    slot_a = b.add(sa, a.cfl, 0.0, 1.0, 4)
+    slot_c = b.add(sc, c.cfl, 0.0, 1.0, 4)
+    sa.set_steady_state(a.steady)
+    before = sa.get_state()
+    sa.snapshot_begin()
+    b.run(2)
+    np.testing.assert_array_equal(sa.snapshot_end(), before)
+    assert b.query(slot_c)["steps"] == 2
+    sc.close()
+    with pytest.raises(z.ZfvError):
+        b.query(slot_c)
+    b.run(5)
+    q = b.query(slot_a)
+    assert q["steps"] == 4 and not q["active"]
+    ref, t_ref, _ = alone(a, 1.0, 4)
+    assert abs(q["t"] - t_ref) <= 1e-15
+    np.testing.assert_array_equal(sa.get_state(), ref)
+    with pytest.raises(z.ZfvError):
+        b.add(sa, a.cfl, 0.0, 1.0, 1)  # already a member
+    b.close()
+    # the batch is gone: its former member can join another one
+    b2 = z.Batch(a.params, cell_capacity=cap, face_capacity=2 * cap, max_instances=4)
+    b2.add(sa, a.cfl, 0.0, 1.0, 1)
+    b2.close()

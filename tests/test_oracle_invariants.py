@@ -357,3 +357,45 @@ def test_time_loop_reaches_the_end_time_and_rejects_as_stated():
     # max_steps stops the loop early
     _, t4, k4, _ = o.advance(wl.u0, wl.cfl, 0.0, 0.01, max_steps=2)
     assert k4 == 2 and t4 < 0.01
+
+
+def test_kept_equilibria_follow_the_counter_and_the_threshold():
+    # LocalReconstruction::recompute_equilibrium
+    # (include/zisa/reconstruction/local_reconstruction.hpp:87-100) on the
+    # oracle, against the oracle that recomputes on every call
+    from conftest import recompute_states, wb_err, with_recompute
+
+    wl = W.isothermal_2d(n=10, order=3, amplitude=1e-3, ghost_layers=0)
+    p = wl.params
+    plain = Oracle(wl.mesh, p)
+    seq = recompute_states(wl.u0, n_calls=5, rel=1e-3, seed=3)
+    ref = [plain.rate(u) for u in seq]
+    # a threshold of zero recomputes on every call: bit for bit the plain rates
+    o = Oracle(wl.mesh, with_recompute(p, 3, 0.0))
+    for u, r in zip(seq, ref):
+        np.testing.assert_array_equal(o.rate(u), r)
+    # no threshold, steps_per_recompute 3: calls 1 and 4 recompute (equal to
+    # the plain rates); call 2 keeps call 1's equilibrium on the same state
+    # (equal as well); calls 3 and 5 keep equilibria of other states (differ
+    # from the plain rates by more than round-off)
+    o = Oracle(wl.mesh, with_recompute(p, 3, 1e300))
+    got = [o.rate(u) for u in seq]
+    for k in (0, 1, 3):
+        np.testing.assert_array_equal(got[k], ref[k])
+    for k in (2, 4):
+        assert max(wb_err(got[k], ref[k], wl.mesh, seq[k], p.gamma)) > 1e-9, k
+    # the threshold measures |((rho - rho_c) / rho_s, (E_int - E_c) / E_s)|
+    # with the kept equilibrium's own cell average (rho_c, E_c) and the kept
+    # scale: scaling the whole state by 1 + eps moves every cell by about
+    # eps * sqrt(2), so eps = 0.8 t recomputes everywhere (where a density-only
+    # or squared measure would not) and eps = 0.6 t keeps every equilibrium
+    t = 1e-4
+    for eps, recomputes in ((0.8 * t, True), (0.6 * t, False)):
+        o = Oracle(wl.mesh, with_recompute(p, 3, t))
+        o.rate(wl.u0)
+        u = wl.u0 * (1.0 + eps)
+        a, b = o.rate(u), plain.rate(u)
+        if recomputes:
+            np.testing.assert_array_equal(a, b)
+        else:
+            assert max(wb_err(a, b, wl.mesh, u, p.gamma)) > 1e-11

@@ -134,6 +134,8 @@ def make_params(
     volume_deg=None,
     moments_deg=-1,
     flux_bc=0,
+    steps_per_recompute=1,
+    recompute_threshold=1e300,
 ) -> Params:
     """Reconstruction parameters in the reference's shape
     (src/zisa/reconstruction/hybrid_weno_params.cpp:26-44: orders {p,2,2,2},
@@ -169,8 +171,8 @@ def make_params(
     p.volume_deg = volume_deg
     p.moments_deg = moments_deg
     p.flux_bc = flux_bc
-    p.steps_per_recompute = 1
-    p.recompute_threshold = 1e300
+    p.steps_per_recompute = steps_per_recompute
+    p.recompute_threshold = recompute_threshold
     return p
 
 

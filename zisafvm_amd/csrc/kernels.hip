@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "setup.hpp"
+#include "batch.hpp"
 #include "stage_math.hpp"
 
 namespace zfv {
@@ -95,12 +96,6 @@ struct QuadGroup : WaveGroup {
     return sel32(v, k);
   }
 };
-
-// groups of NV lanes per 64-wide wavefront (2D: 16 cells, 3D: 12 cells)
-template <int NV>
-constexpr int groups_per_wave() {
-  return 64 / NV;
-}
 
 // k_reconstruct: blocks of `blockDim.x / 64` waves, 64 / NV cells per wave
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
@@ -251,127 +246,52 @@ __global__ __launch_bounds__(256) void k_phi_moments(Img m, double *out) {
 }
 
 // ---------------------------------------------------------------------------
-// Batched execution: many instances of one kernel shape advance together.
-// Each instance keeps its own arrays; a device table of DevInst describes
-// them, and block tables map every launched block to (instance, local block).
-// The graph of one step is captured once for the batch's capacity; adding or
-// removing instances only rewrites the device tables between replays.
+// Batch sizes: the largest batches run on the GPU with results checked are
+// 65,536 cells and 8 members.  A batched run of 64 members and 4.0M cells
+// ended in a GPU memory fault whose cause has not been found (the block
+// tables and the per-block indexing at that size are in bounds on the host
+// emulation), so zfv_batch_create refuses larger batches.
+constexpr int64_t kBatchMaxCells = 65536;
+constexpr int32_t kBatchMaxMembers = 8;
 
-struct DevInst {
-  Img m;
-  double *U, *X, *K, *fpv, *G, *src;
-  double cfl, t, t_end, dt;
-  unsigned long long dt_bits;
-  int active;
-  int steps_done;
-  int max_steps;
-  int pad;
-};
-
-struct BlockRef {
-  int inst;   // -1: no work
-  int local;  // block index inside the instance
-};
+// ---------------------------------------------------------------------------
+// Batched execution (the per-block work is in batch.hpp, shared with the host
+// emulation of the tests).  The graph of one step is captured once for the
+// batch's capacity; adding or removing instances only rewrites the device
+// tables between replays.  Block tables already place neighbouring blocks on
+// one XCD, so the kernels index them by blockIdx directly.
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>
-__global__ __launch_bounds__(256) void kb_reconstruct(DevInst *insts, const BlockRef *blk, int stage) {
+__global__ __launch_bounds__(256) void kb_reconstruct(const DevInst *insts, const BlockRef *blk, int stage) {
   constexpr int NV = DIM + 2;
   constexpr int GPW = groups_per_wave<NV>();
   extern __shared__ double slices[];  // as in k_reconstruct
-  const BlockRef b = blk[blockIdx.x];
-  if (b.inst < 0) return;
-  const DevInst &d = insts[b.inst];
-  if (!d.active) return;
   const int lane = threadIdx.x & 63;
   const int grp = lane / NV;
   const int g = lane - grp * NV;
   const int wave = threadIdx.x >> 6;
-  const int64_t i = (int64_t(b.local) * (blockDim.x >> 6) + wave) * GPW + grp;
-  if (grp >= GPW || i >= d.m.n) return;
-  if (d.m.steady != nullptr && (d.m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
   WaveGroup x{lane - g, &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()]};
-  reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(d.m, i, g, x, stage == 0 ? d.U : d.X, d.fpv, d.src);
+  kb_reconstruct_group<DIM, DOFC, RC, NB, RB, Q, PH>(insts, blk[blockIdx.x], blockDim.x >> 6, wave, grp, g, stage,
+                                                      x);
 }
 
 template <int DIM, int Q>
-__global__ __launch_bounds__(256) void kb_face_flux(DevInst *insts, const BlockRef *blk) {
-  const BlockRef b = blk[blockIdx.x];
-  if (b.inst < 0) return;
-  const DevInst &d = insts[b.inst];
-  if (!d.active) return;
-  const int64_t e = int64_t(b.local) * 256 + threadIdx.x;
-  if (e >= d.m.ne) return;
-  face_flux<DIM, Q>(d.m, e, d.fpv, d.G);
+__global__ __launch_bounds__(256) void kb_face_flux(const DevInst *insts, const BlockRef *blk) {
+  kb_face_flux_thread<DIM, Q>(insts, blk[blockIdx.x], threadIdx.x);
 }
 
 template <int DIM>
-__global__ __launch_bounds__(256) void kb_update(DevInst *insts, const BlockRef *blk, int stage) {
-  constexpr int NV = DIM + 2;
-  const BlockRef b = blk[blockIdx.x];
-  if (b.inst < 0) return;
-  const DevInst &d = insts[b.inst];
-  if (!d.active) return;
-  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
-  const Img &m = d.m;
-  const int64_t n = m.n;
-  if (i >= n) return;
-  const double *Us = stage == 0 ? d.U : d.X;
-  double *Un = stage == 2 ? d.U : d.X;
-  if (m.steady != nullptr && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) {
-    for (int v = 0; v < NV; ++v) {
-      d.K[(int64_t(stage) * NV + v) * n + i] = 0.0;
-      Un[v * n + i] = m.steady[v * n + i];
-    }
-    return;
-  }
-  double L[NV];
-  cell_rate<DIM>(m, i, Us, d.G, d.src, L);
-  // SSP3 next-state coefficients (src/zisa/ode/runge_kutta.cpp:154-165)
-  const double c0 = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 1.0 / 6);
-  const double c1 = stage == 1 ? 0.25 : 1.0 / 6;
-  const double c2 = 2.0 / 3;
-  const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
-  for (int v = 0; v < NV; ++v) {
-    d.K[(int64_t(stage) * NV + v) * n + i] = L[v];
-    double dudt = 0.0;
-    if (stage == 0) {
-      dudt += c0 * L[v];
-    } else if (stage == 1) {
-      dudt += c0 * d.K[(int64_t(0) * NV + v) * n + i];
-      dudt += c1 * L[v];
-    } else {
-      dudt += c0 * d.K[(int64_t(0) * NV + v) * n + i];
-      dudt += c1 * d.K[(int64_t(1) * NV + v) * n + i];
-      dudt += c2 * L[v];
-    }
-    double un = d.U[v * n + i] + d.dt * dudt;
-    if (ghost) un = m.steady[v * n + i];
-    Un[v * n + i] = un;
-  }
+__global__ __launch_bounds__(256) void kb_update(const DevInst *insts, const BlockRef *blk, int stage) {
+  kb_update_thread<DIM>(insts, blk[blockIdx.x], threadIdx.x, stage);
 }
 
 template <int DIM>
 __global__ __launch_bounds__(256) void kb_cfl(DevInst *insts, const BlockRef *blk) {
-  constexpr int NV = DIM + 2;
   const BlockRef b = blk[blockIdx.x];
-  if (b.inst < 0) return;
+  if (b.inst < 0) return;  // the whole block shares b: it leaves together
   DevInst &d = insts[b.inst];
   if (!d.active) return;
-  const int64_t i = int64_t(b.local) * 256 + threadIdx.x;
-  const Img &m = d.m;
-  double r = 1.0e300;
-  if (i < m.n) {
-    const int64_t n = m.n;
-    double u[NV];
-    for (int v = 0; v < NV; ++v) u[v] = d.U[v * n + i];
-    double m2 = u[1] * u[1] + u[2] * u[2];
-    if (DIM == 3) m2 += u[3] * u[3];
-    double v2 = m2 / (u[0] * u[0]);
-    double p = (u[NV - 1] - 0.5 * m2 / u[0]) * (m.gamma - 1.0);
-    double a = sqrt(m.gamma * p / u[0]);
-    r = m.inr[i] / (sqrt(v2) + a);
-    if (!(r > 0.0)) r = 0.0;
-  }
+  double r = kb_cfl_thread<DIM>(d, b, threadIdx.x);
   for (int off = 32; off > 0; off >>= 1) r = fmin(r, __shfl_down(r, off, 64));
   __shared__ double wmin[4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -388,26 +308,12 @@ __global__ __launch_bounds__(256) void kb_cfl(DevInst *insts, const BlockRef *bl
 // per-instance step bookkeeping (one thread per instance)
 __global__ void kb_begin_step(DevInst *insts, int n_inst) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_inst) return;
-  const double big = 1.0e300;
-  unsigned long long bits;
-  memcpy(&bits, &big, sizeof(bits));
-  insts[k].dt_bits = bits;
+  if (k < n_inst) kb_begin_instance(insts[k]);
 }
 
 __global__ void kb_choose_dt(DevInst *insts, int n_inst) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_inst) return;
-  DevInst &d = insts[k];
-  if (!d.active) return;
-  double m;
-  unsigned long long bits = d.dt_bits;
-  memcpy(&m, &bits, sizeof(m));
-  double dt = d.cfl * m;
-  const double rest = d.t_end - d.t;
-  if (dt > rest) dt = rest;
-  d.dt = dt;
-  if (!(dt > 0.0)) d.active = 0;
+  if (k < n_inst) kb_choose_instance(insts[k]);
 }
 
 // A single instance's time loop on the device (zfv_advance): TimeLoop's
@@ -504,12 +410,7 @@ __global__ void k_adv_end(AdvState *a) {
 
 __global__ void kb_end_step(DevInst *insts, int n_inst) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_inst) return;
-  DevInst &d = insts[k];
-  if (!d.active) return;
-  d.t += d.dt;
-  d.steps_done += 1;
-  if (d.t >= d.t_end || d.steps_done >= d.max_steps) d.active = 0;
+  if (k < n_inst) kb_end_instance(insts[k]);
 }
 
 // plain streaming copy (16-byte vectors, grid-stride) for bandwidth calibration
@@ -669,6 +570,9 @@ struct zfv_handle {
   hipStream_t copy_stream = nullptr;
   hipEvent_t snap_ready = nullptr;
   bool snap_pending = false;
+  // the batch this instance is a member of (zfv_batch_add), or null
+  zfv_batch *batch = nullptr;
+  int batch_slot = -1;
 
   template <class T>
   T *alloc(size_t count) {
@@ -701,6 +605,11 @@ struct zfv_handle {
 };
 
 namespace zfv {
+
+// batch membership hooks of the instance calls (defined with the batch)
+static void batch_forget(zfv_batch *b, int slot);
+static void batch_member_steady(zfv_batch *b, int slot, const double *steady);
+static void batch_after_snapshot(zfv_batch *b, hipEvent_t copied);
 
 static int fail(const std::exception &e) {
   g_last_error = e.what();
@@ -749,6 +658,15 @@ static StageCoeffs ssp3_coeffs(int stage) {
   return c;
 }
 
+// the image with stateful equilibria switched off: its launches recompute
+// every cell's equilibrium and leave the kept ones and their counters alone
+static Img stateless(const Img &m) {
+  Img r = m;
+  r.eqc = nullptr;
+  r.eqcnt = nullptr;
+  return r;
+}
+
 // steps run under the frozen BC once a steady state is set
 static int frozen(const zfv_handle *h) { return h->dimg.steady != nullptr ? 1 : 0; }
 
@@ -758,7 +676,7 @@ static int frozen(const zfv_handle *h) { return h->dimg.steady != nullptr ? 1 : 
 // events on the handle's stream; the fastest average wins.  The launches only
 // write the face-point and source buffers.
 static void tune_rec(zfv_handle *h, const double *Ustate) {
-  const Img &m = h->dimg;
+  const Img m = stateless(h->dimg);
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -970,6 +888,8 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.K = params->polytrope_K;
   m.G = params->polytrope_G;
   m.flux_bc = params->flux_bc;
+  m.spr = params->steps_per_recompute;
+  m.eq_threshold = params->recompute_threshold;
   for (int q = 0; q < 16; ++q)
     for (int d = 0; d < 3; ++d) m.shift[q][d] = h->img.shift[3 * q + d];
   if (im.face_w.size() > 8 || im.vol_w.size() > 16) throw ZfvError(ZFV_ERR_INVALID, "quadrature rule too large");
@@ -1026,6 +946,12 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     HIP_CHECK(hipStreamSynchronize(h->stream));
     m.pot = tab;
   }
+  if (params->steps_per_recompute > 1) {
+    // kept equilibria and their call counters (zero: the first call recomputes)
+    m.eqc = h->alloc<double>(size_t(8) * n);
+    m.eqcnt = h->alloc<int32_t>(size_t(n));
+    HIP_CHECK(hipMemset(m.eqcnt, 0, sizeof(int32_t) * size_t(n)));
+  }
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
   *out = h.release();
   return ZFV_OK;
@@ -1033,8 +959,18 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
 }
 
 int zfv_destroy(zfv_handle *h) {
+  if (!h) return ZFV_OK;
+  int rc = ZFV_OK;
+  if (h->batch) {
+    // a member leaves its batch first: the batch must not run on freed arrays
+    try {
+      batch_forget(h->batch, h->batch_slot);
+    } catch (const std::exception &e) {
+      rc = fail(e);
+    }
+  }
   delete h;
-  return ZFV_OK;
+  return rc;
 }
 
 int zfv_set_state(zfv_handle *h, const double *cvars) {
@@ -1072,6 +1008,8 @@ int zfv_snapshot_begin(zfv_handle *h) {
   HIP_CHECK(hipMemcpyAsync(h->snap_dev, h->U, sizeof(double) * count, hipMemcpyDeviceToDevice, h->stream));
   HIP_CHECK(hipEventRecord(h->snap_ready, h->stream));
   HIP_CHECK(hipStreamWaitEvent(h->copy_stream, h->snap_ready, 0));
+  // a batch member's later steps run on the batch's stream: after the copy
+  if (h->batch) batch_after_snapshot(h->batch, h->snap_ready);
   HIP_CHECK(hipMemcpyAsync(h->snap_host, h->snap_dev, sizeof(double) * count, hipMemcpyDeviceToHost,
                            h->copy_stream));
   h->snap_pending = true;
@@ -1099,6 +1037,7 @@ int zfv_set_steady_state(zfv_handle *h, const double *cvars) {
   HIP_CHECK(hipStreamSynchronize(h->stream));
   h->dimg.steady = h->steady;
   drop_graphs(h);  // the captured steps hold the BC and the skip flag
+  if (h->batch) batch_member_steady(h->batch, h->batch_slot, h->steady);
   return ZFV_OK;
   ZFV_API_END
 }
@@ -1323,7 +1262,7 @@ int zfv_get_stencils(zfv_handle *h, int32_t i, int32_t *l2g, int32_t cap, int32_
 
 int zfv_time_kernels(zfv_handle *h, int32_t n_steps, double dt, zfv_kernel_times *out) {
   ZFV_API_BEGIN
-  const Img &m = h->dimg;
+  const Img m = stateless(h->dimg);
   const int cfg = rec_config(h, h->U);
   std::vector<hipEvent_t> ev(2 * 9 * std::max(n_steps, 1));
   for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
@@ -1380,6 +1319,7 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
     per_cell += 8.0 * (16 * (1 + s.dim) + s.nfaces * s.q);  // potential tables (at most)
   }
   if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
+  if (params->steps_per_recompute > 1) per_cell += 8.0 * 8 + 4.0;       // kept equilibria
   double per_face = 8.0 * nv * (2.0 * s.q + 1.0);                      // point values + fluxes
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;
@@ -1472,6 +1412,9 @@ struct zfv_batch {
   hipGraphExec_t graph = nullptr;
   bool dirty = true;
   ~zfv_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (zfv_handle *h : members)
+      if (h) h->batch = nullptr;
     if (graph) (void)hipGraphExecDestroy(graph);
     if (d_inst) (void)hipFree(d_inst);
     if (d_bcell) (void)hipFree(d_bcell);
@@ -1493,23 +1436,11 @@ static bool same_shape(const Shape &a, const Shape &b) {
 // neighbouring cells then share one XCD's L2.
 // The table has as many entries (a multiple of 8) as the members need; the
 // step graph is launched with exactly that many blocks.
-static std::vector<BlockRef> block_table(const std::vector<int64_t> &counts, int64_t per_block, int64_t cap,
-                                         bool xcd_chunks) {
-  std::vector<BlockRef> logical;
-  for (size_t s = 0; s < counts.size(); ++s)
-    for (int64_t k = 0; k < (counts[s] + per_block - 1) / per_block; ++k) logical.push_back({int(s), int(k)});
-  const int64_t size = std::max<int64_t>(8, ((int64_t(logical.size()) + 7) / 8) * 8);
-  if (size > cap) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
-  std::vector<BlockRef> t(size, BlockRef{-1, 0});
-  if (!xcd_chunks) {
-    std::copy(logical.begin(), logical.end(), t.begin());
-    return t;
-  }
-  const int64_t g8 = size / 8;
-  for (int64_t j = 0; j < int64_t(logical.size()); ++j) {
-    const int64_t x = j / g8, k = j % g8;
-    t[8 * k + x] = logical[j];
-  }
+// a block table of at most cap entries, or ZFV_ERR_MEMORY
+static std::vector<BlockRef> checked_table(const std::vector<int64_t> &counts, int64_t per_block, int64_t cap,
+                                           bool xcd_chunks) {
+  auto t = block_table(counts, per_block, cap, xcd_chunks);
+  if (t.empty()) throw ZfvError(ZFV_ERR_MEMORY, "batch capacity exceeded");
   return t;
 }
 
@@ -1538,10 +1469,10 @@ static void batch_upload(zfv_batch *b) {
     b->graph = nullptr;
   }
   b->ph = ph;
-  auto tc = block_table(cells, 256, b->cap_cell_blocks, b->xcd_chunks);
-  auto tf = block_table(faces, 256, b->cap_face_blocks, b->xcd_chunks);
+  auto tc = checked_table(cells, 256, b->cap_cell_blocks, b->xcd_chunks);
+  auto tf = checked_table(faces, 256, b->cap_face_blocks, b->xcd_chunks);
   const int waves = rec_cfg_waves(b->rec_cfg >= 0 ? b->rec_cfg : kRecConfigs - 1);
-  auto tr = block_table(cells, int64_t(waves) * b->L.groups_per_wave, b->cap_rec_blocks, b->xcd_chunks);
+  auto tr = checked_table(cells, int64_t(waves) * b->L.groups_per_wave, b->cap_rec_blocks, b->xcd_chunks);
   if (int64_t(tc.size()) != b->n_cell_blocks || int64_t(tf.size()) != b->n_face_blocks ||
       int64_t(tr.size()) != b->n_rec_blocks) {
     // launch sizes change: the step graph is captured again on the next run
@@ -1566,32 +1497,49 @@ static void batch_upload(zfv_batch *b) {
 // candidate's block table is uploaded and stage-0 reconstructions of all
 // active members are timed (one warm-up, three timed launches).
 static void batch_tune(zfv_batch *b) {
-  hipEvent_t e0, e1;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
-  int best = kRecConfigs - 1;
-  double best_ms = 1e300;
-  for (int c = 0; c < kRecConfigs; ++c) {
-    b->rec_cfg = c;
-    batch_upload(b);
-    const unsigned nr = unsigned(b->n_rec_blocks);
-    b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
-    HIP_CHECK(hipEventRecord(e0, b->stream));
-    for (int k = 0; k < 3; ++k) b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
-    HIP_CHECK(hipEventRecord(e1, b->stream));
-    HIP_CHECK(hipEventSynchronize(e1));
-    HIP_CHECK(hipGetLastError());
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    b->rec_ms[c] = ms / 3.0;
-    if (b->rec_ms[c] < best_ms) {
-      best_ms = b->rec_ms[c];
-      best = c;
+  // the timed launches leave the members' kept equilibria alone
+  std::vector<Img> kept(b->max_inst);
+  for (int s = 0; s < b->max_inst; ++s) {
+    kept[s] = b->h_inst[s].m;
+    b->h_inst[s].m = stateless(kept[s]);
+  }
+  auto restore = [&] {
+    for (int s = 0; s < b->max_inst; ++s) b->h_inst[s].m = kept[s];
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  try {
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int best = kRecConfigs - 1;
+    double best_ms = 1e300;
+    for (int c = 0; c < kRecConfigs; ++c) {
+      b->rec_cfg = c;
+      batch_upload(b);
+      const unsigned nr = unsigned(b->n_rec_blocks);
+      b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
+      HIP_CHECK(hipEventRecord(e0, b->stream));
+      for (int k = 0; k < 3; ++k) b->L.brec[b->ph](b->d_inst, b->d_brec, nr, 0, c, b->stream);
+      HIP_CHECK(hipEventRecord(e1, b->stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      HIP_CHECK(hipGetLastError());
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      b->rec_ms[c] = ms / 3.0;
+      if (b->rec_ms[c] < best_ms) {
+        best_ms = b->rec_ms[c];
+        best = c;
+      }
     }
+    b->rec_cfg = best;
+  } catch (...) {
+    restore();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    throw;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->rec_cfg = best;
+  restore();
   batch_upload(b);
 }
 
@@ -1615,6 +1563,27 @@ static void batch_enqueue_step(zfv_batch *b) {
   hipLaunchKernelGGL(kb_end_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
 }
 
+static void batch_forget(zfv_batch *b, int slot) {
+  batch_sync_down(b);
+  if (b->members[slot]) {
+    b->members[slot]->batch = nullptr;
+    b->members[slot]->batch_slot = -1;
+  }
+  b->members[slot] = nullptr;
+  b->h_inst[slot].active = 0;
+  batch_upload(b);
+}
+
+static void batch_member_steady(zfv_batch *b, int slot, const double *steady) {
+  batch_sync_down(b);
+  b->h_inst[slot].m.steady = steady;
+  batch_upload(b);
+}
+
+static void batch_after_snapshot(zfv_batch *b, hipEvent_t copied) {
+  HIP_CHECK(hipStreamWaitEvent(b->stream, copied, 0));
+}
+
 }  // namespace zfv
 
 extern "C" {
@@ -1623,6 +1592,10 @@ int zfv_batch_create(const zfv_params *params, int64_t cell_capacity, int64_t fa
                      zfv_batch **out) {
   ZFV_API_BEGIN
   if (!params || !out || max_instances < 1 || cell_capacity < 1) throw ZfvError(ZFV_ERR_INVALID, "bad batch arguments");
+  if (cell_capacity > kBatchMaxCells || max_instances > kBatchMaxMembers)
+    throw ZfvError(ZFV_ERR_INVALID, "batch capacity above " + std::to_string(kBatchMaxCells) + " cells or " +
+                                        std::to_string(kBatchMaxMembers) +
+                                        " members: larger batches are refused (an unexplained GPU memory fault)");
   check_device();
   auto b = std::make_unique<zfv_batch>();
   b->shape = shape_for(*params);
@@ -1664,6 +1637,7 @@ int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_e
                   int32_t *slot) {
   ZFV_API_BEGIN
   if (!same_shape(b->shape, h->img.s)) throw ZfvError(ZFV_ERR_INVALID, "instance shape differs from the batch's");
+  if (h->batch) throw ZfvError(ZFV_ERR_INVALID, "the instance is already a member of a batch");
   batch_sync_down(b);
   int s = -1;
   for (int k = 0; k < b->max_inst; ++k)
@@ -1697,6 +1671,8 @@ int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_e
     batch_upload(b);
     throw;
   }
+  h->batch = b;
+  h->batch_slot = s;
   *slot = s;
   return ZFV_OK;
   ZFV_API_END
@@ -1705,10 +1681,7 @@ int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_e
 int zfv_batch_remove(zfv_batch *b, int32_t slot) {
   ZFV_API_BEGIN
   if (slot < 0 || slot >= b->max_inst || !b->members[slot]) throw ZfvError(ZFV_ERR_INVALID, "bad slot");
-  batch_sync_down(b);
-  b->members[slot] = nullptr;
-  b->h_inst[slot].active = 0;
-  batch_upload(b);
+  batch_forget(b, slot);
   return ZFV_OK;
   ZFV_API_END
 }

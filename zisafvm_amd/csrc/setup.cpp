@@ -859,8 +859,11 @@ Shape shape_for(const zfv_params &p) {
                    "flux_bc isentropic (EquilibriumFluxBC) is implemented with isentropic well-balancing only");
   if (!(p.gamma > 1.0)) throw ZfvError(ZFV_ERR_INVALID, "gamma must be > 1");
   if (p.scaling != 0) throw ZfvError(ZFV_ERR_INVALID, "the kernels implement EulerScaling only (scaling = 0)");
-  if (p.steps_per_recompute != 1)
-    throw ZfvError(ZFV_ERR_INVALID, "steps_per_recompute must be 1: the kernels re-solve equilibria every call");
+  if (p.steps_per_recompute < 1) throw ZfvError(ZFV_ERR_INVALID, "steps_per_recompute must be >= 1");
+  if (p.steps_per_recompute > 1 && p.flux_bc == ZFV_FLUX_BC_ISENTROPIC)
+    throw ZfvError(ZFV_ERR_INVALID,
+                   "steps_per_recompute > 1 with flux_bc isentropic is not implemented: EquilibriumFluxBC solves the "
+                   "current equilibrium on every call while the reconstruction keeps its own");
   if (p.equilibrium != ZFV_EQ_NONE && p.gravity == ZFV_GRAVITY_NONE)
     throw ZfvError(ZFV_ERR_INVALID, "a well-balanced equilibrium needs gravity");
   return s;

@@ -60,6 +60,15 @@ struct Img {
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
+  // stateful equilibria (steps_per_recompute > 1), or null: per cell theta
+  // (2), found flag, EulerScaling (rho, a, E_int) and the equilibrium's own
+  // cell average (rho, E) of the last recompute (8 x n), and calls since it
+  // (n); launches that must leave them alone (autotuning, kernel timing) run
+  // with eqc null and recompute
+  double *eqc;
+  int32_t *eqcnt;
+  int spr;               // steps_per_recompute
+  double eq_threshold;   // recompute_threshold
   // physics
   double gamma;
   double eps, exponent;
@@ -626,9 +635,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   if (DIM == 3) msq += u0[3] * u0[3];
   const double rho0 = u0[0];
   const double Eint0 = u0[NV - 1] - 0.5 * msq / rho0;
-  const double cs = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
-  const double scale_g = g == 0 ? rho0 : (g == NV - 1 ? Eint0 : cs);
-  const double inv_g = 1.0 / scale_g;
+  // this call's scale; a stateful instance may keep the last recompute's
+  double sc_rho = rho0, sc_E = Eint0, sc_a = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
   const bool is_rho = g == 0, is_E = g == NV - 1;
 
   double vtx[DIM + 1][3];
@@ -677,6 +685,36 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     double xref[3];
     volume_point<DIM>(m, 0, vtx, xref);
     phi_ref = vol_phi(0, xref);
+  }
+  // Stateful equilibria (LocalReconstruction::recompute_equilibrium,
+  // include/zisa/reconstruction/local_reconstruction.hpp:87-100): between
+  // recomputes (every steps_per_recompute calls) the cell keeps the theta and
+  // the scale of its last recompute, unless its (rho, E_int) has moved from
+  // the kept equilibrium's own cell average by the threshold or more,
+  // measured in the kept scale; a kept equilibrium that was not found
+  // averages to zero there.
+  int32_t calls = 0;
+  bool reuse = false;
+  if (m.eqc != nullptr) {
+    calls = m.eqcnt[i];
+    if (calls % m.spr != 0) {
+      const bool kept_found = m.eqc[2 * n + i] != 0.0;
+      const double kth[2] = {m.eqc[i], m.eqc[n + i]};
+      const double c_rho = m.eqc[3 * n + i], c_a = m.eqc[4 * n + i], c_E = m.eqc[5 * n + i];
+      const double er = m.eqc[6 * n + i], eE = m.eqc[7 * n + i];
+      const double d0 = (rho0 - er) / c_rho, d1 = (Eint0 - eE) / c_E;
+      reuse = !(sqrt(d0 * d0 + d1 * d1) >= m.eq_threshold);
+      if (reuse) {
+        th[0] = kth[0];
+        th[1] = kth[1];
+        eq_found = wb && kept_found;
+        sc_rho = c_rho;
+        sc_a = c_a;
+        sc_E = c_E;
+      }
+    }
+  }
+  if (wb && !reuse) {
     double vol_i = m.vol[i];
     // cell average of the equilibrium with parameters t over this cell
     auto cell_avg = [&](const double *t, double &ar, double &aE) {
@@ -753,6 +791,22 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       }
     }
   }
+  if (m.eqc != nullptr) {
+    xch.sync();  // every lane of the group has read the kept state
+  }
+  if (m.eqc != nullptr && g == 0) {
+    if (!reuse) {
+      m.eqc[i] = th[0];
+      m.eqc[n + i] = th[1];
+      m.eqc[2 * n + i] = eq_found ? 1.0 : 0.0;
+      m.eqc[3 * n + i] = sc_rho;
+      m.eqc[4 * n + i] = sc_a;
+      m.eqc[5 * n + i] = sc_E;
+    }
+    m.eqcnt[i] = reuse ? calls + 1 : 1;
+  }
+  const double scale_g = g == 0 ? sc_rho : (g == NV - 1 ? sc_E : sc_a);
+  const double inv_g = 1.0 / scale_g;
 
   const EqC thc = eq_consts<PH>(m, th, phi_ref);
 
@@ -812,6 +866,12 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   {
     double er = 0.0, eE = 0.0;
     if (eq_sub) own_eq(er, eE);
+    if (m.eqc != nullptr && !reuse && g == 0) {
+      // the recomputed equilibrium's own average (zero if none), which the
+      // next calls measure the threshold against
+      m.eqc[6 * n + i] = er;
+      m.eqc[7 * n + i] = eE;
+    }
     q0 = u0[g];
     if (is_rho) q0 -= er;
     if (is_E) q0 -= eE;
