@@ -166,6 +166,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.mom = im.mom.data();
     m.meta = im.meta.data();
     m.cface = im.cface.data();
+    m.fslot = im.fslot.data();
     m.fperm = im.fperm.data();
     m.steady = nullptr;
     m.phim = nullptr;
@@ -238,7 +239,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
       m.pot = e->pot.data();
     }
     const int nv = im.s.nvars;
-    e->fpv.assign(size_t(2) * im.s.q * nv * std::max<int64_t>(im.ne, 1), 0.0);
+    e->fpv.assign(size_t(im.s.nfaces) * im.s.q * nv * im.n, 0.0);
     e->G.assign(size_t(nv) * std::max<int64_t>(im.ne, 1), 0.0);
     if (p->gravity != 0) e->src.assign(size_t(nv) * im.n, 0.0);
     *out = e.release();
@@ -514,7 +515,7 @@ int zfv_emu_batch_create(void **members, int n, const double *const *u0, const d
       M.U.assign(size_t(nv) * nn, 0.0);
       M.X.assign(size_t(nv) * nn, 0.0);
       M.K.assign(size_t(3) * nv * nn, 0.0);
-      M.fpv.assign(size_t(2) * s.q * nv * ne, 0.0);
+      M.fpv.assign(size_t(s.nfaces) * s.q * nv * nn, 0.0);
       M.G.assign(size_t(nv) * ne, 0.0);
       if (e->m.gravity != 0) M.src.assign(size_t(nv) * nn, 0.0);
       to_soa(im, u0[k], M.U.data());

@@ -869,6 +869,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.mom = h->upload(im.mom);
   m.meta = h->upload(im.meta);
   m.cface = h->upload(im.cface);
+  m.fslot = h->upload(im.fslot);
   m.fperm = h->upload(im.fperm);
   m.steady = nullptr;
   m.phim = nullptr;
@@ -909,7 +910,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   h->U = h->alloc<double>(size_t(nv) * n);
   h->X = h->alloc<double>(size_t(nv) * n);
   h->K = h->alloc<double>(size_t(3) * nv * n);
-  h->fpv = h->alloc<double>(size_t(2) * s.q * nv * std::max<int64_t>(im.ne, 1));
+  h->fpv = h->alloc<double>(size_t(s.nfaces) * s.q * nv * n);  // Q x NV per cell-face slot
   h->G = h->alloc<double>(size_t(nv) * std::max<int64_t>(im.ne, 1));
   if (params->gravity != ZFV_GRAVITY_NONE) h->src = h->alloc<double>(size_t(nv) * n);
   h->dt_bits = h->alloc<unsigned long long>(1);
@@ -1216,7 +1217,7 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
                      4.0 * (1 + s.nfaces + 1) + tables +
                      8.0 * (2.0 * ne / n) * s.q * nv + (h->src ? 8.0 * nv : 0.0);
-  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv);
+  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv + 4.0 * 2);
   const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * frame + 1) + 4.0 + 8.0 * nv * 5;
   info->recon_bytes_per_cell = rec;
   info->bytes_per_cell_stage = rec + face + upd;
@@ -1320,7 +1321,8 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   }
   if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
   if (params->steps_per_recompute > 1) per_cell += 8.0 * 8 + 4.0;       // kept equilibria
-  double per_face = 8.0 * nv * (2.0 * s.q + 1.0);                      // point values + fluxes
+  per_cell += 8.0 * s.nfaces * s.q * nv;                               // face-point values per cell-face slot
+  double per_face = 8.0 * nv + 4.0 * 2;                                // fluxes + the two slots
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;
   ZFV_API_END

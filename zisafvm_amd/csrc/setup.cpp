@@ -1106,6 +1106,15 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
     }
     img.fperm[ip] = fp;
   }
+  if (int64_t(n) * nv > INT32_MAX) throw ZfvError(ZFV_ERR_INVALID, "mesh too large for 32-bit cell-face slots");
+  img.fslot.assign(size_t(2) * std::max<int64_t>(img.ne, 1), 0);
+#pragma omp parallel for schedule(static)
+  for (int ip = 0; ip < n; ++ip)
+    for (int k = 0; k < nv; ++k) {
+      const int32_t cf = img.cface[size_t(k) * N + ip];
+      if (cf & CFACE_EXTERIOR) continue;
+      img.fslot[size_t(cf & CFACE_SIDE_R) * size_t(img.ne) + size_t(cf >> 2)] = ip * nv + k;
+    }
 
   // normalized rules for the kernels
   Rule fr = dim == 2 ? gl_edge(p.edge_deg) : dunavant(p.edge_deg);

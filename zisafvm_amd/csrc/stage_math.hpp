@@ -55,6 +55,7 @@ struct Img {
   const double *mom;
   const int32_t *meta;
   const int32_t *cface;
+  const int32_t *fslot;  // 2 * ne: cell-face slot of each interior face's left / right cell
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
@@ -610,7 +611,9 @@ ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f)
 //   - the face-frame rotation and the gravity source mix momentum components.
 // Every call of bcast sits in control flow that is uniform over the group.
 //
-// U: state, nvars x n SoA.  fpv: [side][q][v] x ne.  src: nvars x n or null.
+// U: state, nvars x n SoA.  fpv: per cell-face slot (i * NF + f) its Q x NV
+// point values, written for interior faces (one group writes its cell's whole
+// block, so cache lines fill within one XCD).  src: nvars x n or null.
 // LDS doubles of one cell group: the NV pseudo-inverse rows of the chunk
 // being accumulated, then the central rows' member equilibrium averages
 // (er, eE per row) for the one-sided rows that share their member
@@ -1217,9 +1220,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         if (g == 1) r = DIM == 2 ? mc[0] * nn[0] + mc[1] * nn[1] : mc[0] * nn[0] + mc[1] * nn[1] + mc[2] * nn[2];
         if (g == 2) r = DIM == 2 ? mc[0] * t1[0] + mc[1] * t1[1] : mc[0] * t1[0] + mc[1] * t1[1] + mc[2] * t1[2];
         if (DIM == 3 && g == 3) r = mc[0] * t2[0] + mc[1] * t2[1] + mc[2] * t2[2];
-        const int64_t e = cf >> 2;
-        const int side = cf & 1;
-        fpv[((side * Q + q) * NV + g) * m.ne + e] = r;
+        fpv[((int64_t(i) * NF + f) * Q + q) * NV + g] = r;
       }
       if (gravity_src && eq_sub && g >= 1 && g <= DIM) {
         double p_eq = eqE[q] * (gamma - 1.0);
@@ -1291,14 +1292,17 @@ template <int DIM, int Q>
 ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, double *__restrict__ G) {
   constexpr int NV = DIM + 2;
   const int64_t ne = m.ne;
+  // the two cells' point values of this face: Q x NV contiguous doubles each
+  const double *fL = fpv + int64_t(m.fslot[e]) * (Q * NV);
+  const double *fR = fpv + int64_t(m.fslot[ne + e]) * (Q * NV);
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     double uL[NV], uR[NV], f[NV];
     for (int v = 0; v < NV; ++v) {
-      uL[v] = fpv[((0 * Q + q) * NV + v) * ne + e];
-      uR[v] = fpv[((1 * Q + q) * NV + v) * ne + e];
+      uL[v] = fL[q * NV + v];
+      uR[v] = fR[q * NV + v];
     }
     if (m.flux == 0) hllc<NV>(uL, uR, m.gamma, f);
     else rusanov<NV>(uL, uR, m.gamma, f);
