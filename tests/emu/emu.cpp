@@ -30,7 +30,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim, pot, frm, eqc;
+  std::vector<double> U, fpv, G, src, phim, pot, frm, eqc, cphi;
   std::vector<int32_t> eqcnt;
 };
 
@@ -226,6 +226,19 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
         store_face_frames<3>(m, i, e->frm.data());
     }
     m.frm = e->frm.data();
+    for (int c = 0; c < 16; ++c)
+      m.shift_phi[c] = im.s.dim == 2 ? phi<2, PH_LIGHT>(m, m.shift[c]) : phi<3, PH_LIGHT>(m, m.shift[c]);
+    if (physics_set(m) == PH_LIGHT && m.eq != 0) {
+      // the corner potentials the GPU path keeps for the light set
+      e->cphi.assign(size_t(im.s.dim + 1) * im.n, 0.0);
+      for (int64_t i = 0; i < im.n; ++i) {
+        if (im.s.dim == 2)
+          store_corner_phi<2>(m, i, e->cphi.data());
+        else
+          store_corner_phi<3>(m, i, e->cphi.data());
+      }
+      m.cphi = e->cphi.data();
+    }
     if (physics_set(m) != PH_LIGHT && p->gravity != ZFV_GRAVITY_NONE) {
       // the potential tables the GPU path keeps for these physics sets
       const int rows = im.s.dim == 2 ? pot_rows<2>(m.qv, im.s.q) : pot_rows<3>(m.qv, im.s.q);

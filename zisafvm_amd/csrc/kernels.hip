@@ -226,6 +226,14 @@ __global__ __launch_bounds__(256) void k_face_frames(Img m, double *tab) {
   store_face_frames<DIM>(m, i, tab);
 }
 
+// the light set's corner potentials
+template <int DIM>
+__global__ __launch_bounds__(256) void k_corner_phi(Img m, double *tab) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  store_corner_phi<DIM>(m, i, tab);
+}
+
 // per-cell potential tables of the cells' own quadrature points
 template <int DIM>
 __global__ __launch_bounds__(256) void k_potential_points(Img m, int nqf, double *tab) {
@@ -936,6 +944,18 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     m.frm = tab;
   }
   h->ph = physics_set(m);
+  for (int c = 0; c < 16; ++c) m.shift_phi[c] = s.dim == 2 ? phi<2, PH_LIGHT>(m, m.shift[c]) : phi<3, PH_LIGHT>(m, m.shift[c]);
+  if (h->ph == PH_LIGHT && m.eq != 0) {
+    // member equilibrium averages read the members' corner potentials
+    double *tab = h->alloc<double>(size_t(s.dim + 1) * n);
+    if (s.dim == 2)
+      hipLaunchKernelGGL(k_corner_phi<2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, tab);
+    else
+      hipLaunchKernelGGL(k_corner_phi<3>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, m, tab);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    m.cphi = tab;
+  }
   if (h->ph != PH_LIGHT && params->gravity != ZFV_GRAVITY_NONE) {
     // potential tables of the cells' own quadrature points
     double *tab = h->alloc<double>(size_t(s.dim == 2 ? pot_rows<2>(m.qv, s.q) : pot_rows<3>(m.qv, s.q)) * n);
@@ -1322,6 +1342,7 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
   if (params->steps_per_recompute > 1) per_cell += 8.0 * 8 + 4.0;       // kept equilibria
   per_cell += 8.0 * s.nfaces * s.q * nv;                               // face-point values per cell-face slot
+  if (params->equilibrium != ZFV_EQ_NONE) per_cell += 8.0 * s.nfaces;  // corner potentials (light set)
   double per_face = 8.0 * nv + 4.0 * 2;                                // fluxes + the two slots
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;

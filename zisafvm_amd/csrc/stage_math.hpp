@@ -61,6 +61,9 @@ struct Img {
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
+  // the potential at every cell's corners ((d + 1) x n; the light set with an
+  // equilibrium, whose axial potential is affine), or null
+  const double *cphi;
   // stateful equilibria (steps_per_recompute > 1), or null: per cell theta
   // (2), found flag, EulerScaling (rho, a, E_int) and the equilibrium's own
   // cell average (rho, E) of the last recompute (8 x n), and calls since it
@@ -81,6 +84,7 @@ struct Img {
   double g, axis[3], rhoC, K, G;
   int flux_bc;
   double shift[16][3];  // periodic translations addressed by the row shift code
+  double shift_phi[16];  // axial potential of each translation (added to cphi)
   // normalized quadrature rules
   int qv;                   // points of the volume rule
   double fw[8], fx[8][3];   // face rule: weights, GL abscissa (2D) or barycentric (3D)
@@ -335,6 +339,14 @@ ZFV_HD void face_frame(const double (*w)[3], double *n, double *t1, double *t2, 
 // vertices for its quadrature points anyway, rebuilds them: measured faster.)
 template <int DIM>
 constexpr int frame_words() { return DIM == 2 ? 3 : 7; }
+
+// writes the light set's potential at cell i's corners (tab: (d + 1) n)
+template <int DIM>
+ZFV_HD void store_corner_phi(const Img &m, int64_t i, double *tab) {
+  double v[DIM + 1][3];
+  load_corners<DIM>(m, i, v);
+  for (int k = 0; k < DIM + 1; ++k) tab[k * m.n + i] = phi<DIM, PH_LIGHT>(m, v[k]);
+}
 
 // writes the frames of cell i's local faces (tab has frame_words (d+1) n entries)
 template <int DIM>
@@ -818,10 +830,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   auto member_avg = [&](const double (*w3)[3], double vol_j, double &er, double &eE) {
     if (PH == PH_LIGHT) {
       // axial gravity: phi is affine, so its point values follow from the
-      // corners; rho_eq = th0 exp(a (phi - phi_ref)) and E_eq = b rho_eq
-      // (b constant), so both averages come from one weighted sum
+      // corner potentials load_member left in w3[k][0]; rho_eq = th0
+      // exp(a (phi - phi_ref)) and E_eq = b rho_eq (b constant), so both
+      // averages come from one weighted sum
       double pv[DIM + 1];
-      for (int k = 0; k < DIM + 1; ++k) pv[k] = phi<DIM, PH>(m, w3[k]);
+      for (int k = 0; k < DIM + 1; ++k) pv[k] = w3[k][0];
       // two points per iteration: independent exponentials interleave; the
       // sum still runs in point order.  An odd last point pairs with itself
       // at weight zero (adding +0 leaves the sum unchanged), so the loop body
@@ -931,6 +944,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     auto load_member = [&](int32_t rw, Member &mb) {
       const int64_t j = rw & ROW_CELL_MASK;
       const int code = rw >> ROW_CELL_BITS;
+      if (PH == PH_LIGHT) {
+        // the member's corner potentials (translated), in w3[k][0]
+        for (int k = 0; k < DIM + 1; ++k) mb.w3[k][0] = m.cphi[k * n + j] + m.shift_phi[code];
+        return;
+      }
       mb.lin = PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
       if (PH == PH_LIN || mb.lin) {
         mb.pm = m.phim[j];
