@@ -45,7 +45,8 @@ struct LaneBarrier {
   std::atomic<long> gen{0};
   std::atomic<bool> broken{false};
   double slot[8] = {0};
-  double lds[128] = {0};  // the group's LDS slice (slice_doubles <= 128)
+  double lds[128] = {0};   // the group's LDS slice (group_lds_doubles <= 128)
+  double wave[640] = {0};  // the row region of its wavefront (wave_lds_doubles <= 640), group 0
   explicit LaneBarrier(int n_) : n(n_) {}
   void wait() {
     if (broken.load()) throw std::runtime_error("lane group broken");
@@ -82,6 +83,8 @@ struct EmuXch {
   }
   int32_t bcast_i(int32_t v, int k) { return int32_t(bcast(double(v), k)); }
   double *lds() { return b->lds; }
+  double *wave_lds() { return b->wave; }
+  int lane0() const { return 0; }  // the emulated group is lane group 0 of its wavefront
   void sync() { b->wait(); }
 };
 

@@ -2,9 +2,10 @@
 // libzfv_hip.so (declared in include/zfv.h).
 //
 // One SSP3 stage = three launches over SoA arrays in HBM:
-//   k_reconstruct  one thread per cell: stencil gather, LSQ pseudo-inverse
-//                  mat-vecs, CWENO/WENO-AO, equilibrium solve (well-balanced),
-//                  point values on the cell's faces (face frame), gravity source
+//   k_reconstruct  one group of nvars lanes per cell (lane g: variable g):
+//                  stencil gather, LSQ pseudo-inverse mat-vecs, CWENO/WENO-AO,
+//                  equilibrium solve (well-balanced), point values on the
+//                  cell's faces (face frame), gravity source
 //   k_face_flux    one thread per interior face: HLLC/Rusanov at the face
 //                  points, integrated once per face
 //   k_update       one thread per cell: atomic-free gather of the cell's face
@@ -53,11 +54,14 @@ __device__ __forceinline__ int64_t xcd_block(unsigned b, unsigned grid) {
 // Cross-lane exchange inside a group of NV consecutive lanes of a wavefront
 // (one cell per group; lane g of the group owns conserved variable g).
 struct WaveGroup {
-  int base;     // first lane of the group
+  int base;       // first lane of the group
   double *slice;  // the group's LDS slice
+  double *wave;   // the wavefront's row region
   __device__ __forceinline__ double bcast(double v, int k) const { return __shfl(v, base + k, 64); }
   __device__ __forceinline__ int32_t bcast_i(int32_t v, int k) const { return __shfl(v, base + k, 64); }
   __device__ __forceinline__ double *lds() const { return slice; }
+  __device__ __forceinline__ double *wave_lds() const { return wave; }
+  __device__ __forceinline__ int lane0() const { return base; }
   // orders the group's LDS writes before its reads (and reads before the
   // next writes): the lanes of a group share a wavefront, whose LDS
   // operations execute in order
@@ -116,12 +120,13 @@ __global__ __launch_bounds__(256) void k_reconstruct(
   // under the frozen BC a ghost cell away from the interior feeds no flux and
   // its own tendency is overwritten: nothing to reconstruct
   if (skip_far_ghosts && (m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
-  double *sl = &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()];
+  double *wv = &slices[wave * wave_block_doubles<DIM, DOFC, RC>()];
+  double *sl = wv + wave_lds_doubles<DIM, DOFC>() + grp * group_lds_doubles<RC>();
   if constexpr (NV == 4) {
-    QuadGroup x{{lane - g, sl}};
+    QuadGroup x{{lane - g, sl, wv}};
     reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
   } else {
-    WaveGroup x{lane - g, sl};
+    WaveGroup x{lane - g, sl, wv};
     reconstruct_lane<DIM, DOFC, RC, NB, RB, Q, PH>(m, i, g, x, U, fpv, src);
   }
 }
@@ -278,7 +283,8 @@ __global__ __launch_bounds__(256) void kb_reconstruct(const DevInst *insts, cons
   const int grp = lane / NV;
   const int g = lane - grp * NV;
   const int wave = threadIdx.x >> 6;
-  WaveGroup x{lane - g, &slices[(wave * GPW + grp) * slice_doubles<DIM, DOFC, RC>()]};
+  double *wv = &slices[wave * wave_block_doubles<DIM, DOFC, RC>()];
+  WaveGroup x{lane - g, wv + wave_lds_doubles<DIM, DOFC>() + grp * group_lds_doubles<RC>(), wv};
   kb_reconstruct_group<DIM, DOFC, RC, NB, RB, Q, PH>(insts, blk[blockIdx.x], blockDim.x >> 6, wave, grp, g, stage,
                                                       x);
 }
@@ -460,7 +466,7 @@ inline int rec_cfg_waves(int cfg) { return 1 << cfg; }
 // LDS of a reconstruction block of `waves` wavefronts: one slice per cell group
 template <int DIM, int DOFC, int RC>
 inline unsigned rec_lds_bytes(int waves) {
-  return unsigned(waves * groups_per_wave<DIM + 2>() * slice_doubles<DIM, DOFC, RC>() * sizeof(double));
+  return unsigned(waves * wave_block_doubles<DIM, DOFC, RC>() * sizeof(double));
 }
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH>

@@ -14,6 +14,7 @@
 // pseudo-inverse P = (A^T A)^{-1} A^T formed once here with the same pivoted
 // LDLT, so a kernel applies one small mat-vec per stencil.
 #include "setup.hpp"
+#include "stage_math.hpp"
 
 #include <algorithm>
 #include <array>
@@ -1054,7 +1055,13 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
 #pragma omp parallel for schedule(static)
   for (int ip = 0; ip < n; ++ip) {
     const int i = img.perm[ip];
-    for (int m = 0; m < s.m; ++m) img.P[size_t(m) * N + ip] = Pc[size_t(i) * s.m + m];
+    // central coefficient a of row r (flat a rc + r) at p_central, the
+    // one-sided rows flat
+    for (int m = 0; m < s.m; ++m) {
+      const int nc = s.dofc - 1;
+      const size_t at = m < nc * s.rc ? size_t(p_central(nc, s.rc, n, m / s.rc, m % s.rc, ip)) : size_t(m) * N + ip;
+      img.P[at] = Pc[size_t(i) * s.m + m];
+    }
     for (int r = 0; r < s.nrows; ++r) {
       const int32_t rc = rows_c[size_t(i) * s.nrows + r];
       img.rows[size_t(r) * N + ip] = img.iperm[rc & ((1 << 27) - 1)] | (rc & ~((1 << 27) - 1));

@@ -626,11 +626,28 @@ ZFV_HD void rusanov(const double *uL, const double *uR, double gamma, double *f)
 // U: state, nvars x n SoA.  fpv: per cell-face slot (i * NF + f) its Q x NV
 // point values, written for interior faces (one group writes its cell's whole
 // block, so cache lines fill within one XCD).  src: nvars x n or null.
-// LDS doubles of one cell group: the NV pseudo-inverse rows of the chunk
-// being accumulated, then the central rows' member equilibrium averages
-// (er, eE per row) for the one-sided rows that share their member
+// LDS of one wavefront: first a row region of wave_lds_doubles holding each
+// group's NV pseudo-inverse rows of NC entries (group c at c NV NC: a stride
+// of NV NC doubles keeps the groups' reads on different banks), then per
+// cell group its central rows' member equilibrium averages (er, eE per row,
+// for the one-sided rows that share their member; group_lds_doubles).
+template <int DIM, int DOFC>
+constexpr int wave_lds_doubles() { return (64 / (DIM + 2)) * (DIM + 2) * (DOFC - 1); }
+template <int RC>
+constexpr int group_lds_doubles() { return 2 * RC; }
 template <int DIM, int DOFC, int RC>
-constexpr int slice_doubles() { return (DIM + 2) * (DOFC - 1) + 2 * RC; }
+constexpr int wave_block_doubles() { return wave_lds_doubles<DIM, DOFC>() + (64 / (DIM + 2)) * group_lds_doubles<RC>(); }
+// the index of central coefficient a of row r of cell i in the pseudo-inverse
+// array (pairs a, a + 1 of one row and cell adjacent, one 16-byte load; an
+// odd last coefficient in its own block); the one-sided rows follow at nc rc n
+struct alignas(16) PPair {
+  double lo, hi;
+};
+ZFV_HD int64_t p_central(int nc, int rc, int64_t n, int a, int r, int64_t i) {
+  const int ncp = nc / 2;
+  if (a < 2 * ncp) return ((int64_t(a >> 1) * rc + r) * n + i) * 2 + (a & 1);
+  return int64_t(2 * ncp) * rc * n + int64_t(r) * n + i;
+}
 
 template <int DIM, int DOFC, int RC, int NB, int RB, int Q, int PH, class X>
 ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const double *__restrict__ U,
@@ -913,8 +930,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // overlaps arithmetic), then loads the row of the pseudo-inverse, which
     // reaches every lane through the group's LDS slice; the averages reach
     // the lanes by wavefront shuffles.
-    double *pl = xch.lds();       // NV rows x NC coefficients
-    double *ceq = pl + NV * NC;   // central rows' member averages (er, eE)
+    double *pl = xch.wave_lds() + xch.lane0() * NC;  // this group's NV rows x NC coefficients
+    double *ceq = xch.lds();                         // central rows' member averages (er, eE)
+    constexpr int NCP = NC / 2;                      // coefficient pairs of a central row
     // scaled deviation of row k of the chunk in this lane's variable
     // (the equilibrium parts enter with factors 1 or 0: u - 1 er = u - er
     // and x - 0 eE = x exactly, so this equals the branchy form bit for bit)
@@ -939,14 +957,18 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double w3[DIM + 1][3];
       double vol;
       double pm, pv;
+      double sphi;  // light set: the translation's potential, added at use
       int lin;
     };
     auto load_member = [&](int32_t rw, Member &mb) {
       const int64_t j = rw & ROW_CELL_MASK;
       const int code = rw >> ROW_CELL_BITS;
       if (PH == PH_LIGHT) {
-        // the member's corner potentials (translated), in w3[k][0]
-        for (int k = 0; k < DIM + 1; ++k) mb.w3[k][0] = m.cphi[k * n + j] + m.shift_phi[code];
+        // the member's corner potentials in w3[k][0] and its translation's
+        // potential, added where they are used (so that the loads stay in
+        // flight until then)
+        for (int k = 0; k < DIM + 1; ++k) mb.w3[k][0] = m.cphi[k * n + j];
+        mb.sphi = m.shift_phi[code];
         return;
       }
       mb.lin = PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
@@ -961,8 +983,15 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       mb.vol = m.vol[j];
     };
     auto member_avg_of = [&](const Member &mb, double &er, double &eE) {
-      if (PH == PH_LIN || (PH == PH_FULL && mb.lin)) linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
-      else member_avg(mb.w3, mb.vol, er, eE);
+      if (PH == PH_LIN || (PH == PH_FULL && mb.lin)) {
+        linear_isentropic_avg(thc, mb.pm, mb.pv, er, eE);
+      } else if (PH == PH_LIGHT) {
+        double w3[DIM + 1][3];
+        for (int k = 0; k < DIM + 1; ++k) w3[k][0] = mb.w3[k][0] + mb.sphi;
+        member_avg(w3, mb.vol, er, eE);
+      } else {
+        member_avg(mb.w3, mb.vol, er, eE);
+      }
     };
     // All rows R = RC + NB RB (central rows, then the one-sided stencils'
     // rows kb RB + rr) in one stream of chunks: the row entries run two chunks
@@ -1014,8 +1043,15 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       }
       double pr[NC];
       if (my_r < RC) {
+        // coefficient pairs in one 16-byte load each
+        const PPair *pp = reinterpret_cast<const PPair *>(m.P);
 #pragma unroll
-        for (int a = 0; a < NC; ++a) pr[a] = m.P[(a * RC + my_r) * n + i];
+        for (int p = 0; p < NCP; ++p) {
+          const PPair v = pp[p_central(NC, RC, n, 2 * p, my_r, i) >> 1];
+          pr[2 * p] = v.lo;
+          pr[2 * p + 1] = v.hi;
+        }
+        if (NC & 1) pr[NC - 1] = m.P[p_central(NC, RC, n, NC - 1, my_r, i)];
       } else {
         const int kb = (my_r - RC) / RB, rr = (my_r - RC) - kb * RB;
 #pragma unroll
