@@ -1292,47 +1292,39 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // the row and face loops above
     load_corners<DIM>(m, i, vtx);
     const double vol_i = m.vol[i];
-    double cs_ = 0.0;
-    for (int q0 = 0; q0 < m.qv; q0 += NV) {
-      // equilibrium values at the points q0 .. q0 + NV - 1, one per lane
-      double my_r = 0.0, my_E = 0.0;
-      if (eq_sub && q0 + g < m.qv) {
-        double x[3];
-        volume_point<DIM>(m, q0 + g, vtx, x);
-        eq_at_phi<PH>(m, thc, vol_phi(q0 + g, x), my_r, my_E);
-      }
+    // The volume terms, -(rho - rho_eq) grad phi for the momentum and
+    // -m . grad phi for the energy, need only the lanes' own deviations (the
+    // equilibrium is at rest and differs from the state only in rho and E;
+    // without one the deviation is the state): every lane sums -delta d_d phi
+    // for every d (the density lane's sums feed the momentum lanes, momentum
+    // lane d's d-th sum the energy lane), exchanged once, after the points.
+    double acc[DIM];
 #pragma unroll 1
-      for (int k = 0; k < NV; ++k) {
-        const int q = q0 + k;
-        double r_eq = 0.0, E_eq = 0.0;
-        if (eq_sub) {
-          r_eq = xch.bcast(my_r, k);
-          E_eq = xch.bcast(my_E, k);
-        }
-        if (q >= m.qv) break;  // uniform over the group
-        double x[3];
-        volume_point<DIM>(m, q, vtx, x);
-        double u, dl;
-        eval(x, r_eq, E_eq, u, dl);
-        double gp[3];
-        if (tab) {
-          for (int d = 0; d < DIM; ++d) gp[d] = m.pot[pot_index_grad<DIM>(m, q, d) + i];
-        } else {
-          grad_phi<DIM, PH>(m, x, gp);
-        }
-        const double drho = xch.bcast(wb ? dl : u, 0);
-        double mc[3];
-        for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
-        double sv = 0.0;
-        if (g >= 1 && g <= DIM) sv = -drho * gp[g - 1];
-        if (is_E) {
-          double mg = 0.0;
-          for (int d = 0; d < DIM; ++d) mg += mc[d] * gp[d];
-          sv = -mg;
-        }
-        double w = vol_i * m.vw[q];
-        cs_ = (q == 0) ? w * sv : cs_ + w * sv;
+    for (int q = 0; q < m.qv; ++q) {
+      double x[3];
+      volume_point<DIM>(m, q, vtx, x);
+      double u, dl;
+      eval(x, 0.0, 0.0, u, dl);
+      double gp[3];
+      if (tab) {
+        for (int d = 0; d < DIM; ++d) gp[d] = m.pot[pot_index_grad<DIM>(m, q, d) + i];
+      } else {
+        grad_phi<DIM, PH>(m, x, gp);
       }
+      const double w = vol_i * m.vw[q];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const double t = w * (-dl * gp[d]);
+        acc[d] = (q == 0) ? t : acc[d] + t;
+      }
+    }
+    double cs_ = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double a = xch.bcast(acc[d], 0);       // the density lane's sum for direction d
+      const double bm = xch.bcast(acc[d], 1 + d);  // momentum lane d's sum for its own direction
+      if (g == 1 + d) cs_ = a;
+      if (is_E) cs_ = (d == 0) ? bm : cs_ + bm;
     }
     double sum = wb ? (s_acc + 1.0 * cs_) : 1.0 * cs_;
     src[g * n + i] = sum / vol_i;
