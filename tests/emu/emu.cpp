@@ -30,7 +30,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim, pot, frm, eqc, cphi;
+  std::vector<double> U, fpv, G, src, phim, pot, frm, ffr, eqc, cphi;
   std::vector<int32_t> eqcnt;
 };
 
@@ -229,6 +229,14 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
         store_face_frames<3>(m, i, e->frm.data());
     }
     m.frm = e->frm.data();
+    e->ffr.assign(size_t(im.s.dim == 2 ? 2 : 6) * std::max<int64_t>(im.ne, 1), 0.0);
+    for (int64_t f = 0; f < im.ne; ++f) {
+      if (im.s.dim == 2)
+        store_face_table<2>(m, f, e->ffr.data());
+      else
+        store_face_table<3>(m, f, e->ffr.data());
+    }
+    m.ffr = e->ffr.data();
     for (int c = 0; c < 16; ++c)
       m.shift_phi[c] = im.s.dim == 2 ? phi<2, PH_LIGHT>(m, m.shift[c]) : phi<3, PH_LIGHT>(m, m.shift[c]);
     if (physics_set(m) == PH_LIGHT && m.eq != 0) {
@@ -255,7 +263,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
       m.pot = e->pot.data();
     }
     const int nv = im.s.nvars;
-    e->fpv.assign(size_t(im.s.nfaces) * im.s.q * nv * im.n, 0.0);
+    e->fpv.assign(size_t(im.s.nfaces) * ((im.s.q * nv + 1) & ~1) * im.n, 0.0);
     e->G.assign(size_t(nv) * std::max<int64_t>(im.ne, 1), 0.0);
     if (p->gravity != 0) e->src.assign(size_t(nv) * im.n, 0.0);
     *out = e.release();
@@ -531,7 +539,7 @@ int zfv_emu_batch_create(void **members, int n, const double *const *u0, const d
       M.U.assign(size_t(nv) * nn, 0.0);
       M.X.assign(size_t(nv) * nn, 0.0);
       M.K.assign(size_t(3) * nv * nn, 0.0);
-      M.fpv.assign(size_t(s.nfaces) * s.q * nv * nn, 0.0);
+      M.fpv.assign(size_t(s.nfaces) * ((s.q * nv + 1) & ~1) * nn, 0.0);
       M.G.assign(size_t(nv) * ne, 0.0);
       if (e->m.gravity != 0) M.src.assign(size_t(nv) * nn, 0.0);
       to_soa(im, u0[k], M.U.data());

@@ -231,6 +231,14 @@ __global__ __launch_bounds__(256) void k_face_frames(Img m, double *tab) {
   store_face_frames<DIM>(m, i, tab);
 }
 
+// per interior face its frame, in face order
+template <int DIM>
+__global__ __launch_bounds__(256) void k_face_table(Img m, double *tab) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  store_face_table<DIM>(m, e, tab);
+}
+
 // the light set's corner potentials
 template <int DIM>
 __global__ __launch_bounds__(256) void k_corner_phi(Img m, double *tab) {
@@ -924,7 +932,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   h->U = h->alloc<double>(size_t(nv) * n);
   h->X = h->alloc<double>(size_t(nv) * n);
   h->K = h->alloc<double>(size_t(3) * nv * n);
-  h->fpv = h->alloc<double>(size_t(s.nfaces) * s.q * nv * n);  // Q x NV per cell-face slot
+  h->fpv = h->alloc<double>(size_t(s.nfaces) * ((s.q * nv + 1) & ~1) * n);  // Q x NV per cell-face slot, even
   h->G = h->alloc<double>(size_t(nv) * std::max<int64_t>(im.ne, 1));
   if (params->gravity != ZFV_GRAVITY_NONE) h->src = h->alloc<double>(size_t(nv) * n);
   h->dt_bits = h->alloc<unsigned long long>(1);
@@ -948,6 +956,19 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(h->stream));
     m.frm = tab;
+  }
+  {
+    // the interior faces' frames in face order, for the face kernel
+    double *tab = h->alloc<double>(size_t(s.dim == 2 ? 2 : 6) * std::max<int64_t>(im.ne, 1));
+    if (im.ne > 0) {
+      if (s.dim == 2)
+        hipLaunchKernelGGL(k_face_table<2>, dim3(blocks_for(im.ne, 256)), dim3(256), 0, h->stream, m, tab);
+      else
+        hipLaunchKernelGGL(k_face_table<3>, dim3(blocks_for(im.ne, 256)), dim3(256), 0, h->stream, m, tab);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    m.ffr = tab;
   }
   h->ph = physics_set(m);
   for (int c = 0; c < 16; ++c) m.shift_phi[c] = s.dim == 2 ? phi<2, PH_LIGHT>(m, m.shift[c]) : phi<3, PH_LIGHT>(m, m.shift[c]);
@@ -1243,7 +1264,7 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
                      4.0 * (1 + s.nfaces + 1) + tables +
                      8.0 * (2.0 * ne / n) * s.q * nv + (h->src ? 8.0 * nv : 0.0);
-  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv + 4.0 * 2);
+  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv + 4.0 * 2 + 8.0 * (s.dim == 2 ? 2 : 6));
   const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * frame + 1) + 4.0 + 8.0 * nv * 5;
   info->recon_bytes_per_cell = rec;
   info->bytes_per_cell_stage = rec + face + upd;
@@ -1347,9 +1368,9 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   }
   if (mesh->ghost_mask) per_cell += 8.0 * nv;                          // steady state
   if (params->steps_per_recompute > 1) per_cell += 8.0 * 8 + 4.0;       // kept equilibria
-  per_cell += 8.0 * s.nfaces * s.q * nv;                               // face-point values per cell-face slot
+  per_cell += 8.0 * s.nfaces * ((s.q * s.nvars + 1) & ~1);            // face-point values per cell-face slot
   if (params->equilibrium != ZFV_EQ_NONE) per_cell += 8.0 * s.nfaces;  // corner potentials (light set)
-  double per_face = 8.0 * nv + 4.0 * 2;                                // fluxes + the two slots
+  double per_face = 8.0 * nv + 4.0 * 2 + 8.0 * (s.dim == 2 ? 2 : 6);  // fluxes, the two slots, the frame
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;
   ZFV_API_END

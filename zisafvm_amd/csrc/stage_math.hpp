@@ -61,6 +61,7 @@ struct Img {
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
+  const double *ffr;     // per interior face its frame, t1 (2D) or n, t1 (3D), x ne
   // the potential at every cell's corners ((d + 1) x n; the light set with an
   // equilibrium, whose axial potential is affine), or null
   const double *cphi;
@@ -339,6 +340,10 @@ ZFV_HD void face_frame(const double (*w)[3], double *n, double *t1, double *t2, 
 // vertices for its quadrature points anyway, rebuilds them: measured faster.)
 template <int DIM>
 constexpr int frame_words() { return DIM == 2 ? 3 : 7; }
+// doubles of one cell-face slot of face-point values: Q x NV rounded up to
+// even, so that every slot starts on 16 bytes
+template <int DIM, int Q>
+constexpr int slot_doubles() { return (Q * (DIM + 2) + 1) & ~1; }
 
 // writes the light set's potential at cell i's corners (tab: (d + 1) n)
 template <int DIM>
@@ -388,6 +393,43 @@ ZFV_HD void load_face_frame(const Img &m, int64_t i, int f, double *n, double *t
     for (int d = 0; d < 3; ++d) n[d] = p[int64_t(d) * m.n];
     for (int d = 0; d < 3; ++d) t1[d] = p[int64_t(3 + d) * m.n];
     area = p[int64_t(6) * m.n];
+  }
+  t2[0] = n[1] * t1[2] - n[2] * t1[1];
+  t2[1] = n[2] * t1[0] - n[0] * t1[2];
+  t2[2] = n[0] * t1[1] - n[1] * t1[0];
+}
+
+// Per interior face its frame in face order (the face kernel reads it
+// coalesced): written once at setup from the left cell's frame table entry
+template <int DIM>
+ZFV_HD void store_face_table(const Img &m, int64_t e, double *tab) {
+  constexpr int NF = DIM + 1;
+  const int64_t s = m.fslot[e];
+  double n[3], t1[3], t2[3], area;
+  load_face_frame<DIM>(m, s / NF, int(s % NF), n, t1, t2, area);
+  if (DIM == 2) {
+    tab[e] = t1[0];
+    tab[m.ne + e] = t1[1];
+  } else {
+    for (int d = 0; d < 3; ++d) tab[int64_t(d) * m.ne + e] = n[d];
+    for (int d = 0; d < 3; ++d) tab[int64_t(3 + d) * m.ne + e] = t1[d];
+  }
+}
+
+// the frame of interior face e from the face table (the same values, and the
+// same derived n (2D) and t2, as load_face_frame)
+template <int DIM>
+ZFV_HD void face_table_frame(const Img &m, int64_t e, double *n, double *t1, double *t2) {
+  if (DIM == 2) {
+    t1[0] = m.ffr[e];
+    t1[1] = m.ffr[m.ne + e];
+    t1[2] = 0.0;
+    n[0] = t1[1];
+    n[1] = -t1[0];
+    n[2] = t1[2];
+  } else {
+    for (int d = 0; d < 3; ++d) n[d] = m.ffr[int64_t(d) * m.ne + e];
+    for (int d = 0; d < 3; ++d) t1[d] = m.ffr[int64_t(3 + d) * m.ne + e];
   }
   t2[0] = n[1] * t1[2] - n[2] * t1[1];
   t2[1] = n[2] * t1[0] - n[0] * t1[2];
@@ -1267,14 +1309,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         face_point<DIM>(m, q, w3, x);
         double u, dl;
         eval(x, eqr[q], eqE[q], u, dl);
-        // rotate into the face frame: momentum lanes need every component
-        double mc[3];
-        for (int d = 0; d < DIM; ++d) mc[d] = xch.bcast(u, 1 + d);
-        double r = u;
-        if (g == 1) r = DIM == 2 ? mc[0] * nn[0] + mc[1] * nn[1] : mc[0] * nn[0] + mc[1] * nn[1] + mc[2] * nn[2];
-        if (g == 2) r = DIM == 2 ? mc[0] * t1[0] + mc[1] * t1[1] : mc[0] * t1[0] + mc[1] * t1[1] + mc[2] * t1[2];
-        if (DIM == 3 && g == 3) r = mc[0] * t2[0] + mc[1] * t2[1] + mc[2] * t2[2];
-        fpv[((int64_t(i) * NF + f) * Q + q) * NV + g] = r;
+        // in the global frame: the face kernel rotates the momentum into the
+        // face frame, with every component in one thread
+        fpv[(int64_t(i) * NF + f) * slot_doubles<DIM, Q>() + q * NV + g] = u;
       }
       if (gravity_src && eq_sub && g >= 1 && g <= DIM) {
         double p_eq = eqE[q] * (gamma - 1.0);
@@ -1337,19 +1374,42 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 template <int DIM, int Q>
 ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, double *__restrict__ G) {
   constexpr int NV = DIM + 2;
+  constexpr int NF = DIM + 1;
   const int64_t ne = m.ne;
-  // the two cells' point values of this face: Q x NV contiguous doubles each
-  const double *fL = fpv + int64_t(m.fslot[e]) * (Q * NV);
-  const double *fR = fpv + int64_t(m.fslot[ne + e]) * (Q * NV);
+  // the two cells' point values of this face (Q x NV doubles at the start of
+  // each slot, momentum in the global frame), in 16-byte loads
+  constexpr int SQ = slot_doubles<DIM, Q>();
+  double bL[SQ], bR[SQ];
+  const PPair *pL = reinterpret_cast<const PPair *>(fpv + int64_t(m.fslot[e]) * SQ);
+  const PPair *pR = reinterpret_cast<const PPair *>(fpv + int64_t(m.fslot[ne + e]) * SQ);
+#pragma unroll
+  for (int k = 0; k < SQ / 2; ++k) {
+    const PPair l = pL[k], r = pR[k];
+    bL[2 * k] = l.lo;
+    bL[2 * k + 1] = l.hi;
+    bR[2 * k] = r.lo;
+    bR[2 * k + 1] = r.hi;
+  }
+  // the face frame (the left cell's vertex order, as both cells see it)
+  double nn[3], t1[3], t2[3];
+  face_table_frame<DIM>(m, e, nn, t1, t2);
+  auto rotate = [&](double *u) {
+    const double mc[3] = {u[1], u[2], DIM == 3 ? u[3] : 0.0};
+    u[1] = DIM == 2 ? mc[0] * nn[0] + mc[1] * nn[1] : mc[0] * nn[0] + mc[1] * nn[1] + mc[2] * nn[2];
+    u[2] = DIM == 2 ? mc[0] * t1[0] + mc[1] * t1[1] : mc[0] * t1[0] + mc[1] * t1[1] + mc[2] * t1[2];
+    if (DIM == 3) u[3] = mc[0] * t2[0] + mc[1] * t2[1] + mc[2] * t2[2];
+  };
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     double uL[NV], uR[NV], f[NV];
     for (int v = 0; v < NV; ++v) {
-      uL[v] = fL[q * NV + v];
-      uR[v] = fR[q * NV + v];
+      uL[v] = bL[q * NV + v];
+      uR[v] = bR[q * NV + v];
     }
+    rotate(uL);
+    rotate(uR);
     if (m.flux == 0) hllc<NV>(uL, uR, m.gamma, f);
     else rusanov<NV>(uL, uR, m.gamma, f);
     for (int v = 0; v < NV; ++v) acc[v] = acc[v] + m.fw[q] * f[v];
