@@ -442,8 +442,8 @@ ZFV_HD void face_point(const Img &m, int q, const double (*w)[3], double *x) {
   if (DIM == 2) {
     double xi = m.fx[q][0];
     double a = 0.5 - 0.5 * xi, b = 0.5 + 0.5 * xi;
-    x[0] = a * w[0][0] + b * w[1][0];
-    x[1] = a * w[0][1] + b * w[1][1];
+    x[0] = fma(b, w[1][0], a * w[0][0]);
+    x[1] = fma(b, w[1][1], a * w[0][1]);
     x[2] = 0.0;
   } else {
     for (int d = 0; d < 3; ++d) x[d] = (m.fx[q][0] * w[0][d] + m.fx[q][1] * w[1][d]) + m.fx[q][2] * w[2][d];
@@ -454,7 +454,7 @@ template <int DIM>
 ZFV_HD void volume_point(const Img &m, int q, const double (*v)[3], double *x) {
   for (int d = 0; d < DIM; ++d) {
     double s = m.vx[q][0] * v[0][d];
-    for (int k = 1; k < DIM + 1; ++k) s = s + m.vx[q][k] * v[k][d];
+    for (int k = 1; k < DIM + 1; ++k) s = fma(m.vx[q][k], v[k][d], s);
     x[d] = s;
   }
   if (DIM == 2) x[2] = 0.0;
@@ -539,7 +539,7 @@ ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, d
       if (DIM == 2) {
         const int n = kx + ky;
         const int i = (n + 1) * n / 2 + ky;
-        const double b = pxk * pyk - c[i];
+        const double b = fma(pxk, pyk, -c[i]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) px[v] = fma(a[i][v], b, px[v]);
       } else {
@@ -550,7 +550,7 @@ ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, d
           const int d3 = n3 < 0 ? 0 : (n3 + 1) * (n3 + 2) * (n3 + 3) / 6;
           const int n2 = ky + kz;
           const int i = d3 + (n2 + 1) * n2 / 2 + kz;
-          const double b = pxk * pyk * pzk - c[i];
+          const double b = fma(pxk * pyk, pzk, -c[i]);
 #pragma unroll
           for (int v = 0; v < NV; ++v) px[v] = fma(a[i][v], b, px[v]);
           pzk *= X[2];
@@ -900,7 +900,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       // is the only copy of the exponential.
       auto point = [&](int q) {
         double ph = m.vx[q][0] * pv[0];
-        for (int k = 1; k < DIM + 1; ++k) ph = ph + m.vx[q][k] * pv[k];
+        for (int k = 1; k < DIM + 1; ++k) ph = fma(m.vx[q][k], pv[k], ph);
         return exp((ph - thc.phi_ref) * thc.a);
       };
       double se = 0.0;
@@ -908,8 +908,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         const bool pair = q + 1 < m.qv;
         const double e0 = point(q), e1 = point(pair ? q + 1 : q);
         const double w1 = pair ? m.vw[q + 1] : 0.0;
-        se = (q == 0) ? m.vw[q] * e0 : se + m.vw[q] * e0;
-        se = se + w1 * e1;
+        se = (q == 0) ? m.vw[q] * e0 : fma(m.vw[q], e0, se);
+        se = fma(w1, e1, se);
       }
       er = thc.th0 * se;
       eE = er * thc.b;
@@ -980,8 +980,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // and x - 0 eE = x exactly, so this equals the branchy form bit for bit)
     const double f_rho = (wb && is_rho) ? 1.0 : 0.0, f_E = (wb && is_E) ? 1.0 : 0.0;
     auto deviation = [&](double u, double er, double eE) {
-      const double d = (u - f_rho * er) - f_E * eE;
-      return d * inv_g - q0;
+      const double d = fma(-f_E, eE, fma(-f_rho, er, u));
+      return fma(d, inv_g, -q0);
     };
     // a member's equilibrium averages as lane k computed them; in the light
     // set the energy average is the density average times the cell constant
@@ -1317,7 +1317,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
         double p_eq = eqE[q] * (gamma - 1.0);
         double w = area * m.fw[q];
         double t = p_eq * ((sgn - bc_sub) * nn[g - 1]);
-        fs = (q == 0) ? w * t : fs + w * t;
+        fs = (q == 0) ? w * t : fma(w, t, fs);
       }
     }
     if (gravity_src && wb) s_acc += 1.0 * fs;
