@@ -30,7 +30,7 @@ thread_local std::string g_err;
 struct Emu {
   HostImage img;
   Img m{};
-  std::vector<double> U, fpv, G, src, phim, pot, frm, ffr, eqc, cphi;
+  std::vector<double> U, fpv, G, src, phim, pot, frm, ffr, eqc, cphi, smom;
   std::vector<int32_t> eqcnt;
 };
 
@@ -261,6 +261,21 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
           potential_points<3>(m, im.s.q, i, e->pot.data());
       }
       m.pot = e->pot.data();
+    }
+    if (p->gravity != ZFV_GRAVITY_NONE) {
+      // the source moments the GPU path keeps
+      e->smom.assign(size_t(im.s.dofc) * im.s.dim * im.n, 0.0);
+      for (int64_t i = 0; i < im.n; ++i) {
+        if (im.s.dim == 3)
+          store_source_moments<3, 10>(m, i, e->smom.data());
+        else if (im.s.dofc == 10)
+          store_source_moments<2, 10>(m, i, e->smom.data());
+        else if (im.s.dofc == 6)
+          store_source_moments<2, 6>(m, i, e->smom.data());
+        else
+          store_source_moments<2, 3>(m, i, e->smom.data());
+      }
+      m.smom = e->smom.data();
     }
     const int nv = im.s.nvars;
     e->fpv.assign(size_t(im.s.nfaces) * ((im.s.q * nv + 1) & ~1) * im.n, 0.0);

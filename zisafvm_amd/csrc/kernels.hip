@@ -239,6 +239,14 @@ __global__ __launch_bounds__(256) void k_face_table(Img m, double *tab) {
   store_face_table<DIM>(m, e, tab);
 }
 
+// the gravity source moments of every cell
+template <int DIM, int DOFC>
+__global__ __launch_bounds__(256) void k_source_moments(Img m, double *tab) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  store_source_moments<DIM, DOFC>(m, i, tab);
+}
+
 // the light set's corner potentials
 template <int DIM>
 __global__ __launch_bounds__(256) void k_corner_phi(Img m, double *tab) {
@@ -453,6 +461,7 @@ struct Launcher {
   void (*update)(const Img &, StageCoeffs, int, double, const double *, const double *, const double *,
                  const double *, const double *, double *, double *, hipStream_t);
   void (*cfl)(const Img &, const double *, unsigned long long *, hipStream_t);
+  void (*source_moments)(const Img &, double *, hipStream_t);  // setup: Img::smom
   // batched variants: grid = number of block-table entries
   void (*brec[3])(DevInst *, const BlockRef *, unsigned, int, int cfg, hipStream_t);
   void (*bface)(DevInst *, const BlockRef *, unsigned, hipStream_t);
@@ -534,6 +543,9 @@ Launcher make_launcher() {
   };
   l.bcfl = [](DevInst *d, const BlockRef *b, unsigned nb, hipStream_t s) {
     hipLaunchKernelGGL((kb_cfl<DIM>), dim3(nb), dim3(256), 0, s, d, b);
+  };
+  l.source_moments = [](const Img &m, double *tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_source_moments<DIM, DOFC>), dim3(blocks_for(m.n, 256)), dim3(256), 0, s, m, tab);
   };
   l.dim = DIM;
   l.groups_per_wave = groups_per_wave<DIM + 2>();
@@ -983,6 +995,13 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     m.ffr = tab;
   }
   h->ph = physics_set(m);
+  if (params->gravity != ZFV_GRAVITY_NONE) {
+    double *tab = h->alloc<double>(size_t(s.dofc) * s.dim * n);
+    h->L.source_moments(m, tab, h->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    m.smom = tab;
+  }
   for (int c = 0; c < 16; ++c) m.shift_phi[c] = s.dim == 2 ? phi<2, PH_LIGHT>(m, m.shift[c]) : phi<3, PH_LIGHT>(m, m.shift[c]);
   if (h->ph == PH_LIGHT && m.eq != 0) {
     // member equilibrium averages read the members' corner potentials
@@ -1382,6 +1401,7 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   if (params->steps_per_recompute > 1) per_cell += 8.0 * 8 + 4.0;       // kept equilibria
   per_cell += 8.0 * s.nfaces * ((s.q * s.nvars + 1) & ~1);            // face-point values per cell-face slot
   if (params->equilibrium != ZFV_EQ_NONE) per_cell += 8.0 * s.nfaces;  // corner potentials (light set)
+  if (params->gravity != ZFV_GRAVITY_NONE) per_cell += 8.0 * s.dofc * s.dim;  // source moments
   double per_face = 8.0 * nv + 4.0 * 2 + 8.0 * (s.dim == 2 ? 2 : 6);  // fluxes, the two slots, the frame
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;

@@ -62,6 +62,9 @@ struct Img {
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
   const double *ffr;     // per interior face its frame, t1 (2D) or n, t1 (3D), x ne
+  // gravity: per cell vol sum_q vw_q b_alpha(x_q) d_d phi(x_q) over its volume
+  // points, (alpha DIM + d) x n (source_moments), or null
+  const double *smom;
   // the potential at every cell's corners ((d + 1) x n; the light set with an
   // equilibrium, whose axial potential is affine), or null
   const double *cphi;
@@ -562,6 +565,36 @@ ZFV_HD void poly_eval(const double (*a)[NV], const double *c, const double *X, d
   }
 }
 
+// the basis of poly_eval at X, in its order: b[i] = X^alpha_i - c[i]
+template <int DIM, int DEG>
+ZFV_HD void poly_basis(const double *c, const double *X, double *b) {
+  double pxk = 1.0;
+#pragma unroll
+  for (int kx = 0; kx <= DEG; ++kx) {
+    double pyk = 1.0;
+#pragma unroll
+    for (int ky = 0; ky <= DEG - kx; ++ky) {
+      if (DIM == 2) {
+        const int n = kx + ky;
+        b[(n + 1) * n / 2 + ky] = fma(pxk, pyk, -c[(n + 1) * n / 2 + ky]);
+      } else {
+        double pzk = 1.0;
+#pragma unroll
+        for (int kz = 0; kz <= DEG - kx - ky; ++kz) {
+          const int n3 = kx + ky + kz - 1;
+          const int d3 = n3 < 0 ? 0 : (n3 + 1) * (n3 + 2) * (n3 + 3) / 6;
+          const int n2 = ky + kz;
+          const int i = d3 + (n2 + 1) * n2 / 2 + kz;
+          b[i] = fma(pxk * pyk, pzk, -c[i]);
+          pzk *= X[2];
+        }
+      }
+      pyk *= X[1];
+    }
+    pxk *= X[0];
+  }
+}
+
 template <int DIM>
 constexpr int dof_of(int deg) {
   return DIM == 2 ? (deg + 1) * (deg + 2) / 2 : (deg + 1) * (deg + 2) * (deg + 3) / 6;
@@ -572,6 +605,40 @@ constexpr int deg_of(int dof) {
   int d = 0;
   while (dof_of<DIM>(d + 1) <= dof) ++d;
   return d;
+}
+
+// The gravity source moments of cell i: M[alpha][d] = vol sum_q vw_q
+// b_alpha(x_q) d_d phi(x_q) with the volume rule, the polynomial basis of the
+// cell's reconstruction (poly_basis) and the potential's gradient at the
+// points.  The volume terms of the source, sum_q w_q delta(x_q) d_d phi(x_q)
+// for delta = scale sum_alpha a_alpha b_alpha, are then scale sum_alpha
+// a_alpha M[alpha][d]: the same quadrature, summed over the points once at
+// setup instead of per call.
+template <int DIM, int DOFC>
+ZFV_HD void store_source_moments(const Img &m, int64_t i, double *tab) {
+  constexpr int DEG = deg_of<DIM>(DOFC);
+  const int64_t n = m.n;
+  double vtx[DIM + 1][3];
+  load_corners<DIM>(m, i, vtx);
+  double cm[DOFC];
+  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
+  const double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
+  const double inv_len = 1.0 / m.len[i];
+  const double vol_i = m.vol[i];
+  double acc[DOFC][DIM];
+  for (int q = 0; q < m.qv; ++q) {
+    double x[3], gp[3], b[DOFC];
+    volume_point<DIM>(m, q, vtx, x);
+    grad_phi<DIM, PH_FULL>(m, x, gp);
+    const double Xs[3] = {(x[0] - xc[0]) * inv_len, (x[1] - xc[1]) * inv_len,
+                          DIM == 3 ? (x[2] - xc[2]) * inv_len : 0.0};
+    poly_basis<DIM, DEG>(cm, Xs, b);
+    const double w = vol_i * m.vw[q];
+    for (int al = 0; al < DOFC; ++al)
+      for (int d = 0; d < DIM; ++d) acc[al][d] = (q == 0) ? w * (b[al] * gp[d]) : fma(w, b[al] * gp[d], acc[al][d]);
+  }
+  for (int al = 0; al < DOFC; ++al)
+    for (int d = 0; d < DIM; ++d) tab[(int64_t(al) * DIM + d) * n + i] = acc[al][d];
 }
 
 // ---- ideal gas / fluxes -------------------------------------------------------
@@ -1325,42 +1392,29 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   }
 
   if (gravity_src) {
-    // the cell's corners again (from cache): not held in registers through
-    // the row and face loops above
-    load_corners<DIM>(m, i, vtx);
     const double vol_i = m.vol[i];
     // The volume terms, -(rho - rho_eq) grad phi for the momentum and
     // -m . grad phi for the energy, need only the lanes' own deviations (the
     // equilibrium is at rest and differs from the state only in rho and E;
-    // without one the deviation is the state): every lane sums -delta d_d phi
-    // for every d (the density lane's sums feed the momentum lanes, momentum
-    // lane d's d-th sum the energy lane), exchanged once, after the points.
+    // without one the deviation is the state): every lane forms -sum_q w_q
+    // delta(x_q) d_d phi(x_q) for every d from its polynomial and the cell's
+    // source moments (store_source_moments); the density lane's sums feed
+    // the momentum lanes, momentum lane d's d-th sum the energy lane.
     double acc[DIM];
-#pragma unroll 1
-    for (int q = 0; q < m.qv; ++q) {
-      double x[3];
-      volume_point<DIM>(m, q, vtx, x);
-      double u, dl;
-      eval(x, 0.0, 0.0, u, dl);
-      double gp[3];
-      if (tab) {
-        for (int d = 0; d < DIM; ++d) gp[d] = m.pot[pot_index_grad<DIM>(m, q, d) + i];
-      } else {
-        grad_phi<DIM, PH>(m, x, gp);
-      }
-      const double w = vol_i * m.vw[q];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const double t = w * (-dl * gp[d]);
-        acc[d] = (q == 0) ? t : acc[d] + t;
-      }
-    }
+    for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+#pragma unroll
+    for (int al = 0; al < DOFC; ++al)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) acc[d] = fma(a[al], m.smom[(int64_t(al) * DIM + d) * n + i], acc[d]);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) acc[d] = -scale_g * acc[d];
     double cs_ = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      const double a = xch.bcast(acc[d], 0);       // the density lane's sum for direction d
+      const double rs = xch.bcast(acc[d], 0);      // the density lane's sum for direction d
       const double bm = xch.bcast(acc[d], 1 + d);  // momentum lane d's sum for its own direction
-      if (g == 1 + d) cs_ = a;
+      if (g == 1 + d) cs_ = rs;
       if (is_E) cs_ = (d == 0) ? bm : cs_ + bm;
     }
     double sum = wb ? (s_acc + 1.0 * cs_) : 1.0 * cs_;
