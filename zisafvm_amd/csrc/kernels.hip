@@ -1290,8 +1290,11 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   const double nv = s.nvars;
   const double frame = s.dim == 2 ? frame_words<2>() : frame_words<3>();
   const Img &dm = h->dimg;
-  const double tables = (dm.pot ? 8.0 * (s.dim == 2 ? pot_rows<2>(dm.qv, s.q) : pot_rows<3>(dm.qv, s.q)) : 0.0) +
-                        (dm.phim ? 16.0 : 0.0);
+  // tables: the potential at the cell's own volume and face points (its
+  // gradient rows are not read: the source moments hold the volume terms),
+  // the potential moments, the source moments (read outside the light set)
+  const double tables = (dm.pot ? 8.0 * (dm.qv + s.nfaces * s.q) : 0.0) + (dm.phim ? 16.0 : 0.0) +
+                        (dm.smom && h->ph != PH_LIGHT ? 8.0 * s.dofc * s.dim : 0.0);
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
                      4.0 * (1 + s.nfaces + 1) + tables +
                      8.0 * (2.0 * ne / n) * s.q * nv + (h->src ? 8.0 * nv : 0.0);

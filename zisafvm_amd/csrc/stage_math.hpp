@@ -1397,18 +1397,36 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // -m . grad phi for the energy, need only the lanes' own deviations (the
     // equilibrium is at rest and differs from the state only in rho and E;
     // without one the deviation is the state): every lane forms -sum_q w_q
-    // delta(x_q) d_d phi(x_q) for every d from its polynomial and the cell's
-    // source moments (store_source_moments); the density lane's sums feed
-    // the momentum lanes, momentum lane d's d-th sum the energy lane.
+    // delta(x_q) d_d phi(x_q) for every d; the density lane's sums feed the
+    // momentum lanes, momentum lane d's d-th sum the energy lane.
     double acc[DIM];
+    if (PH == PH_LIGHT) {
+      // constant gradient (axial gravity): the quadrature over the points,
+      // cheaper than reading the source moments
+      load_corners<DIM>(m, i, vtx);
+#pragma unroll 1
+      for (int q = 0; q < m.qv; ++q) {
+        double x[3];
+        volume_point<DIM>(m, q, vtx, x);
+        double u, dl;
+        eval(x, 0.0, 0.0, u, dl);
+        double gp[3];
+        grad_phi<DIM, PH>(m, x, gp);
+        const double w = vol_i * m.vw[q];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+        for (int d = 0; d < DIM; ++d) acc[d] = (q == 0) ? w * (-dl * gp[d]) : fma(w, -dl * gp[d], acc[d]);
+      }
+    } else {
+      // from the polynomial and the cell's source moments (store_source_moments)
 #pragma unroll
-    for (int al = 0; al < DOFC; ++al)
+      for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) acc[d] = fma(a[al], m.smom[(int64_t(al) * DIM + d) * n + i], acc[d]);
+      for (int al = 0; al < DOFC; ++al)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) acc[d] = -scale_g * acc[d];
+        for (int d = 0; d < DIM; ++d) acc[d] = fma(a[al], m.smom[(int64_t(al) * DIM + d) * n + i], acc[d]);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) acc[d] = -scale_g * acc[d];
+    }
     double cs_ = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
