@@ -229,7 +229,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
         store_face_frames<3>(m, i, e->frm.data());
     }
     m.frm = e->frm.data();
-    e->ffr.assign(size_t(im.s.dim == 2 ? 2 : 6) * std::max<int64_t>(im.ne, 1), 0.0);
+    e->ffr.assign(size_t(im.s.dim == 2 ? frame_words<2>() : frame_words<3>()) * std::max<int64_t>(im.ne, 1), 0.0);
     for (int64_t f = 0; f < im.ne; ++f) {
       if (im.s.dim == 2)
         store_face_table<2>(m, f, e->ffr.data());

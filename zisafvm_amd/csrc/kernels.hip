@@ -983,7 +983,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   }
   {
     // the interior faces' frames in face order, for the face kernel
-    double *tab = h->alloc<double>(size_t(s.dim == 2 ? 2 : 6) * std::max<int64_t>(im.ne, 1));
+    double *tab = h->alloc<double>(size_t(s.dim == 2 ? frame_words<2>() : frame_words<3>()) * std::max<int64_t>(im.ne, 1));
     if (im.ne > 0) {
       if (s.dim == 2)
         hipLaunchKernelGGL(k_face_table<2>, dim3(blocks_for(im.ne, 256)), dim3(256), 0, h->stream, m, tab);
@@ -1298,8 +1298,8 @@ int zfv_get_info(zfv_handle *h, zfv_info *info) {
   const double rec = 8.0 * s.m + 4.0 * s.nrows + 8.0 * nv + 8.0 * (s.nfaces * s.dim + s.dim + 2 + s.nmom) +
                      4.0 * (1 + s.nfaces + 1) + tables +
                      8.0 * (2.0 * ne / n) * s.q * nv + (h->src ? 8.0 * nv : 0.0);
-  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv + 4.0 * 2 + 8.0 * (s.dim == 2 ? 2 : 6));
-  const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 * (s.nfaces * frame + 1) + 4.0 + 8.0 * nv * 5;
+  const double face = (ne / n) * (8.0 * 2 * s.q * nv + 8.0 * nv + 4.0 * 2 + 8.0 * frame);
+  const double upd = 4.0 * s.nfaces + 8.0 * s.nfaces * nv + 8.0 + 4.0 + 8.0 * nv * 5;
   info->recon_bytes_per_cell = rec;
   info->bytes_per_cell_stage = rec + face + upd;
   int maxs = 0;
@@ -1405,7 +1405,7 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   per_cell += 8.0 * s.nfaces * ((s.q * s.nvars + 1) & ~1);            // face-point values per cell-face slot
   if (params->equilibrium != ZFV_EQ_NONE) per_cell += 8.0 * s.nfaces;  // corner potentials (light set)
   if (params->gravity != ZFV_GRAVITY_NONE) per_cell += 8.0 * s.dofc * s.dim;  // source moments
-  double per_face = 8.0 * nv + 4.0 * 2 + 8.0 * (s.dim == 2 ? 2 : 6);  // fluxes, the two slots, the frame
+  double per_face = 8.0 * nv + 4.0 * 2 + 8.0 * frame;  // fluxes, the two slots, the frame and area
   *bytes = int64_t(n * per_cell + ne * per_face) + (1 << 20);
   return ZFV_OK;
   ZFV_API_END

@@ -61,7 +61,7 @@ struct Img {
   const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
-  const double *ffr;     // per interior face its frame, t1 (2D) or n, t1 (3D), x ne
+  const double *ffr;     // per interior face its frame and area, t1, area (2D) or n, t1, area (3D), x ne
   // gravity: per cell vol sum_q vw_q b_alpha(x_q) d_d phi(x_q) over its volume
   // points, (alpha DIM + d) x n (source_moments), or null
   const double *smom;
@@ -413,26 +413,30 @@ ZFV_HD void store_face_table(const Img &m, int64_t e, double *tab) {
   if (DIM == 2) {
     tab[e] = t1[0];
     tab[m.ne + e] = t1[1];
+    tab[2 * m.ne + e] = area;
   } else {
     for (int d = 0; d < 3; ++d) tab[int64_t(d) * m.ne + e] = n[d];
     for (int d = 0; d < 3; ++d) tab[int64_t(3 + d) * m.ne + e] = t1[d];
+    tab[int64_t(6) * m.ne + e] = area;
   }
 }
 
 // the frame of interior face e from the face table (the same values, and the
 // same derived n (2D) and t2, as load_face_frame)
 template <int DIM>
-ZFV_HD void face_table_frame(const Img &m, int64_t e, double *n, double *t1, double *t2) {
+ZFV_HD void face_table_frame(const Img &m, int64_t e, double *n, double *t1, double *t2, double &area) {
   if (DIM == 2) {
     t1[0] = m.ffr[e];
     t1[1] = m.ffr[m.ne + e];
     t1[2] = 0.0;
+    area = m.ffr[2 * m.ne + e];
     n[0] = t1[1];
     n[1] = -t1[0];
     n[2] = t1[2];
   } else {
     for (int d = 0; d < 3; ++d) n[d] = m.ffr[int64_t(d) * m.ne + e];
     for (int d = 0; d < 3; ++d) t1[d] = m.ffr[int64_t(3 + d) * m.ne + e];
+    area = m.ffr[int64_t(6) * m.ne + e];
   }
   t2[0] = n[1] * t1[2] - n[2] * t1[1];
   t2[1] = n[2] * t1[0] - n[0] * t1[2];
@@ -1443,6 +1447,23 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 // ---- the per-face flux ----------------------------------------------------------
 // G[v][e] = sum_q w_q F(uL_q, uR_q) in the face frame, w_q the normalized
 // face weights (the face measure is applied by the cell update).
+// a flux g given in the face frame (normal, t1, t2 components of the
+// momentum) in the global frame
+template <int DIM>
+ZFV_HD void world_flux(const double *g, const double *nn, const double *t1, const double *t2, double *wv) {
+  constexpr int NV = DIM + 2;
+  wv[0] = g[0];
+  wv[NV - 1] = g[NV - 1];
+  if (DIM == 2) {
+    wv[1] = g[1] * nn[0] + g[2] * t1[0];
+    wv[2] = g[1] * nn[1] + g[2] * t1[1];
+  } else {
+    wv[1] = g[1] * nn[0] + g[2] * t1[0] + g[3] * t2[0];
+    wv[2] = g[1] * nn[1] + g[2] * t1[1] + g[3] * t2[1];
+    wv[3] = g[1] * nn[2] + g[2] * t1[2] + g[3] * t2[2];
+  }
+}
+
 template <int DIM, int Q>
 ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, double *__restrict__ G) {
   constexpr int NV = DIM + 2;
@@ -1463,8 +1484,8 @@ ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, d
     bR[2 * k + 1] = r.hi;
   }
   // the face frame (the left cell's vertex order, as both cells see it)
-  double nn[3], t1[3], t2[3];
-  face_table_frame<DIM>(m, e, nn, t1, t2);
+  double nn[3], t1[3], t2[3], area;
+  face_table_frame<DIM>(m, e, nn, t1, t2, area);
   auto rotate = [&](double *u) {
     const double mc[3] = {u[1], u[2], DIM == 3 ? u[3] : 0.0};
     u[1] = DIM == 2 ? mc[0] * nn[0] + mc[1] * nn[1] : mc[0] * nn[0] + mc[1] * nn[1] + mc[2] * nn[2];
@@ -1486,7 +1507,11 @@ ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, d
     else rusanov<NV>(uL, uR, m.gamma, f);
     for (int v = 0; v < NV; ++v) acc[v] = acc[v] + m.fw[q] * f[v];
   }
-  for (int v = 0; v < NV; ++v) G[v * ne + e] = acc[v];
+  // back to the global frame, times the face's measure: the flux through the
+  // face, which the left cell loses and the right cell gains (cell_rate)
+  double wv[NV];
+  world_flux<DIM>(acc, nn, t1, t2, wv);
+  for (int v = 0; v < NV; ++v) G[v * ne + e] = area * wv[v];
 }
 
 // ---- the per-cell rate of change -------------------------------------------------
@@ -1506,45 +1531,33 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
     // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC; the
     // equilibrium flux BC (flux_bc 2) is part of the source term
     if (exterior && (m.flux_bc != 1 || (cf & 1))) continue;
+    if (!exterior) {
+      // the face kernel's flux through the face (global frame, times area)
+      const int64_t e = cf >> 2;
+      const double sgn = (cf & 1) ? 1.0 : -1.0;
+      for (int v = 0; v < NV; ++v) acc[v] += sgn * G[v * m.ne + e];
+      continue;
+    }
+    // FluxBC: physical flux of the cell average in the face frame
     double nn[3], t1[3], t2[3], area;
     load_face_frame<DIM>(m, i, f, nn, t1, t2, area);
-    double g[NV];
-    double sgn;
-    if (!exterior) {
-      const int64_t e = cf >> 2;
-      for (int v = 0; v < NV; ++v) g[v] = G[v * m.ne + e];
-      sgn = (cf & 1) ? 1.0 : -1.0;
-    } else {
-      // FluxBC: physical flux of the cell average in the face frame
-      double u[NV], r[NV];
-      for (int v = 0; v < NV; ++v) u[v] = U[v * n + i];
-      r[0] = u[0];
-      r[NV - 1] = u[NV - 1];
-      if (DIM == 2) {
-        r[1] = u[1] * nn[0] + u[2] * nn[1];
-        r[2] = u[1] * t1[0] + u[2] * t1[1];
-      } else {
-        r[1] = u[1] * nn[0] + u[2] * nn[1] + u[3] * nn[2];
-        r[2] = u[1] * t1[0] + u[2] * t1[1] + u[3] * t1[2];
-        r[3] = u[1] * t2[0] + u[2] * t2[1] + u[3] * t2[2];
-      }
-      euler_flux<NV>(r, pressure_of<NV>(r, m.gamma), g);
-      sgn = -1.0;
-      // the flux BC uses the face measure directly (weights sum to one)
-    }
-    // back to world frame
-    double wv[NV];
-    wv[0] = g[0];
-    wv[NV - 1] = g[NV - 1];
+    double g[NV], u[NV], r[NV];
+    for (int v = 0; v < NV; ++v) u[v] = U[v * n + i];
+    r[0] = u[0];
+    r[NV - 1] = u[NV - 1];
     if (DIM == 2) {
-      wv[1] = g[1] * nn[0] + g[2] * t1[0];
-      wv[2] = g[1] * nn[1] + g[2] * t1[1];
+      r[1] = u[1] * nn[0] + u[2] * nn[1];
+      r[2] = u[1] * t1[0] + u[2] * t1[1];
     } else {
-      wv[1] = g[1] * nn[0] + g[2] * t1[0] + g[3] * t2[0];
-      wv[2] = g[1] * nn[1] + g[2] * t1[1] + g[3] * t2[1];
-      wv[3] = g[1] * nn[2] + g[2] * t1[2] + g[3] * t2[2];
+      r[1] = u[1] * nn[0] + u[2] * nn[1] + u[3] * nn[2];
+      r[2] = u[1] * t1[0] + u[2] * t1[1] + u[3] * t1[2];
+      r[3] = u[1] * t2[0] + u[2] * t2[1] + u[3] * t2[2];
     }
-    for (int v = 0; v < NV; ++v) acc[v] += sgn * (area * wv[v]);
+    euler_flux<NV>(r, pressure_of<NV>(r, m.gamma), g);
+    // the flux BC uses the face measure directly (weights sum to one)
+    double wv[NV];
+    world_flux<DIM>(g, nn, t1, t2, wv);
+    for (int v = 0; v < NV; ++v) acc[v] += -1.0 * (area * wv[v]);
   }
   const double vol = m.vol[i];
   for (int v = 0; v < NV; ++v) L[v] = acc[v] / vol;
