@@ -93,8 +93,7 @@ typedef struct zfv_params {
                                      autotuning and zfv_time_kernels launches recompute every
                                      cell and leave the kept state alone */
   int32_t scaling;                /* 0 EulerScaling (the experiments' choice), 1 UnityScaling
-                                     (include/zisa/model/characteristic_scale.hpp:14-46);
-                                     the kernels implement EulerScaling */
+                                     (include/zisa/model/characteristic_scale.hpp:14-46) */
   double recompute_threshold;
 } zfv_params;
 

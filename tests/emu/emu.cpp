@@ -180,6 +180,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     m.exponent = p->exponent;
     for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
     m.mode = p->reconstruction;
+    m.scaling = p->scaling;
     m.flux = p->flux;
     m.eq = p->equilibrium;
     m.gravity = p->gravity;

@@ -126,7 +126,7 @@ def test_enumerated_fields_are_range_checked():
     mesh = z.lattice_mesh_2d(6)
     assert _estimate(mesh, z.make_params()) > 0
     bad = [("flux_bc", 3), ("flux_bc", -1), ("flux", 2), ("reconstruction", 2), ("equilibrium", 3), ("gravity", 4),
-           ("gamma", 1.0), ("steps_per_recompute", 0)]
+           ("gamma", 1.0), ("steps_per_recompute", 0), ("scaling", 2), ("scaling", -1)]
     for field, value in bad:
         p = z.make_params()
         setattr(p, field, value)
@@ -157,6 +157,10 @@ def test_enumerated_fields_are_range_checked():
         assert "steps_per_recompute" in str(e)
     else:
         raise AssertionError("accepted steps_per_recompute 2 with the isentropic flux BC")
+    # both of the reference's scalings (EulerScaling, UnityScaling)
+    p = z.make_params()
+    p.scaling = 1
+    assert _estimate(mesh, p) == _estimate(mesh, z.make_params())
     # kept equilibria: 8 doubles and a counter per cell on top
     assert _estimate(mesh, z.make_params(steps_per_recompute=3)) == _estimate(mesh, z.make_params()) + 68 * mesh.n_cells
 

@@ -57,6 +57,27 @@ def test_isothermal_well_balanced(amp):
     assert max(errs) <= TOL, errs
 
 
+@pytest.mark.parametrize("case", ["vortex", "isothermal"])
+def test_unity_scaling(case):
+    # UnityScaling (include/zisa/model/characteristic_scale.hpp:39-46): the
+    # deviations enter the reconstruction unscaled, which moves the WENO
+    # weights away from the EulerScaling result (by far more than the
+    # tolerance: the scales of these states are within a factor 3 of one)
+    wl = W.vortex_2d(n=12) if case == "vortex" else W.isothermal_2d(n=14, amplitude=1e-3)
+    p = wl.params
+    p.scaling = 1
+    o, e = Oracle(wl.mesh, p), Emu(wl.mesh, p)
+    Lo, Le = o.rate(wl.u0), e.rate(wl.u0)
+    if case == "vortex":
+        errs = rel_err(Le, Lo)
+    else:
+        errs = wb_err(Le, Lo, wl.mesh, wl.u0, p.gamma)
+    assert max(errs) <= TOL, errs
+    p.scaling = 0
+    L_euler = Emu(wl.mesh, p).rate(wl.u0)
+    assert max(rel_err(L_euler, Lo)) > 10 * TOL
+
+
 def test_polytrope_3d():
     wl = W.polytrope_3d(n=12, amplitude=1e-3)
     o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)

@@ -113,6 +113,21 @@ def test_isothermal_well_balanced_rate(gpu, amp):
     assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("case", ["vortex", "isothermal"])
+def test_unity_scaling_rate(gpu, case):
+    # UnityScaling (include/zisa/model/characteristic_scale.hpp:39-46)
+    from conftest import wb_err
+
+    wl = W.vortex_2d(n=16) if case == "vortex" else W.isothermal_2d(n=16, amplitude=1e-3)
+    p = wl.params
+    p.scaling = 1
+    s, o = z.Solver(wl.mesh, p), Oracle(wl.mesh, p)
+    Lg, Lo = s.rate(wl.u0), o.rate(wl.u0)
+    errs = rel_err(Lg, Lo) if case == "vortex" else wb_err(Lg, Lo, wl.mesh, wl.u0, p.gamma)
+    print("unity scaling", case, "error per variable", errs)
+    assert max(errs) <= TOL
+
+
 def test_polytrope_3d_rate(gpu):
     from conftest import wb_err
 

@@ -926,6 +926,7 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
   m.exponent = params->exponent;
   for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
   m.mode = params->reconstruction;
+  m.scaling = params->scaling;
   m.flux = params->flux;
   m.eq = params->equilibrium;
   m.gravity = params->gravity;

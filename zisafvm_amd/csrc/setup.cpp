@@ -859,7 +859,8 @@ Shape shape_for(const zfv_params &p) {
     throw ZfvError(ZFV_ERR_INVALID,
                    "flux_bc isentropic (EquilibriumFluxBC) is implemented with isentropic well-balancing only");
   if (!(p.gamma > 1.0)) throw ZfvError(ZFV_ERR_INVALID, "gamma must be > 1");
-  if (p.scaling != 0) throw ZfvError(ZFV_ERR_INVALID, "the kernels implement EulerScaling only (scaling = 0)");
+  if (p.scaling != 0 && p.scaling != 1)
+    throw ZfvError(ZFV_ERR_INVALID, "scaling must be 0 (EulerScaling) or 1 (UnityScaling)");
   if (p.steps_per_recompute < 1) throw ZfvError(ZFV_ERR_INVALID, "steps_per_recompute must be >= 1");
   if (p.steps_per_recompute > 1 && p.flux_bc == ZFV_FLUX_BC_ISENTROPIC)
     throw ZfvError(ZFV_ERR_INVALID,

@@ -82,6 +82,7 @@ struct Img {
   double eps, exponent;
   double lw[5];  // normalized linear weights
   int mode;      // 0 CWENO-AO, 1 WENO-AO
+  int scaling;   // 0 EulerScaling, 1 UnityScaling
   int flux;      // 0 HLLC, 1 Rusanov
   int eq;        // 0 none, 1 isentropic, 2 isothermal
   int gravity;   // 0 none, 1 axial, 2 radial, 3 polytrope
@@ -780,8 +781,11 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   if (DIM == 3) msq += u0[3] * u0[3];
   const double rho0 = u0[0];
   const double Eint0 = u0[NV - 1] - 0.5 * msq / rho0;
-  // this call's scale; a stateful instance may keep the last recompute's
+  // this call's scale (UnityScaling: all ones,
+  // include/zisa/model/characteristic_scale.hpp:39-46); a stateful instance
+  // may keep the last recompute's
   double sc_rho = rho0, sc_E = Eint0, sc_a = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
+  if (m.scaling == 1) sc_rho = sc_E = sc_a = 1.0;
   const bool is_rho = g == 0, is_E = g == NV - 1;
 
   double vtx[DIM + 1][3];
