@@ -268,6 +268,13 @@ int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_e
 int zfv_batch_remove(zfv_batch *b, int32_t slot);
 /* n_steps graph replays; members that reach t_end / max_steps stop. */
 int zfv_batch_run(zfv_batch *b, int32_t n_steps);
+/* zfv_batch_run in two halves, for callers that keep several batches in
+ * flight at once (each batch has its own stream, so batches of different
+ * shapes overlap on the device): launch enqueues n_steps steps and returns,
+ * wait blocks until they have run.  Every other call on the batch or on one
+ * of its members waits for steps in flight first. */
+int zfv_batch_launch(zfv_batch *b, int32_t n_steps);
+int zfv_batch_wait(zfv_batch *b);
 int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps_done, int32_t *active);
 /* Block-to-XCD mapping: 1 (default) gives each XCD a contiguous eighth of the
  * cells so stencil gathers share one L2; 0 is the hardware round robin. */

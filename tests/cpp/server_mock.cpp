@@ -179,6 +179,10 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
   return ZFV_OK;
 }
 
+// the mock's steps run at launch; wait has nothing left to wait for
+int zfv_batch_launch(zfv_batch *b, int32_t n_steps) { return zfv_batch_run(b, n_steps); }
+int zfv_batch_wait(zfv_batch *) { return ZFV_OK; }
+
 int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps, int32_t *active) {
   if (slot < 0 || slot >= b->max_inst || !b->m[slot].h) fail("query of an empty slot");
   const MockMember &mm = b->m[slot];

@@ -111,3 +111,45 @@ def test_member_steady_state_snapshot_and_destroy_inside_a_batch(gpu):
     b2 = z.Batch(a.params, cell_capacity=cap, face_capacity=2 * cap, max_instances=4)
     b2.add(sa, a.cfl, 0.0, 1.0, 1)
     b2.close()
+
+
+def test_batches_of_two_shapes_in_flight_together(gpu):
+    # a 2D and a 3D batch launched one after the other and waited for
+    # together (their streams overlap on the device): every member ends
+    # bit-identical to its own time loop; calls on a batch or a member with
+    # steps in flight wait for them first
+    w2 = [W.vortex_2d(n=12, beta=4.0), W.isothermal_2d(n=10, amplitude=1e-3, order=3)]
+    w3 = [W.polytrope_3d(n=10, amplitude=1e-3)]
+    batches, members = [], []
+    for wls in (w2, w3):
+        cap = sum(w.mesh.n_cells for w in wls)
+        b = z.Batch(wls[0].params, cell_capacity=cap, face_capacity=4 * cap, max_instances=4)
+        for w in wls:
+            s = z.Solver(w.mesh, w.params)
+            s.set_state(w.u0)
+            if w.steady is not None:
+                s.set_steady_state(w.steady)
+            members.append((w, s, b, b.add(s, w.cfl, 0.0, 1.0, 4)))
+        batches.append(b)
+    for b in batches:
+        b.launch(2)
+    for b in batches:
+        b.launch(2)
+    # a query with steps in flight waits for them
+    w, s, b, slot = members[0]
+    assert b.query(slot)["steps"] == 4
+    for b in batches:
+        b.wait()
+    for w, s, b, slot in members:
+        ref, t_ref, k_ref = alone(w, 1.0, 4)
+        q = b.query(slot)
+        assert q["steps"] == k_ref == 4 and abs(q["t"] - t_ref) <= 1e-15
+        np.testing.assert_array_equal(s.get_state(), ref)
+    # a member's state read with its batch in flight waits for the batch
+    w, s, b, slot = members[-1]
+    b.launch(1)
+    u = s.get_state()
+    b.wait()
+    np.testing.assert_array_equal(u, s.get_state())
+    for b in batches:
+        b.close()

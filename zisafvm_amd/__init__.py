@@ -223,6 +223,8 @@ def load_library(path: str = LIB_PATH):
                                   ctypes.POINTER(ctypes.c_int32)]),
         "zfv_batch_remove": (c_int, [vp, ctypes.c_int32]),
         "zfv_batch_run": (c_int, [vp, ctypes.c_int32]),
+        "zfv_batch_launch": (c_int, [vp, ctypes.c_int32]),
+        "zfv_batch_wait": (c_int, [vp]),
         "zfv_batch_query": (c_int, [vp, ctypes.c_int32, dp, dp, ip, ip]),
         "zfv_batch_set_xcd_chunks": (c_int, [vp, ctypes.c_int32]),
         "zfv_tune": (c_int, [vp, ctypes.c_int32, ctypes.POINTER(Tuning)]),
@@ -489,6 +491,13 @@ class Batch:
 
     def run(self, n_steps: int):
         _check(self.lib.zfv_batch_run(self.b, n_steps))
+
+    def launch(self, n_steps: int):
+        """Enqueue n_steps steps and return (the batch's own stream)."""
+        _check(self.lib.zfv_batch_launch(self.b, n_steps))
+
+    def wait(self):
+        _check(self.lib.zfv_batch_wait(self.b))
 
     def query(self, slot: int):
         t, dt = ctypes.c_double(), ctypes.c_double()

@@ -644,6 +644,7 @@ namespace zfv {
 static void batch_forget(zfv_batch *b, int slot);
 static void batch_member_steady(zfv_batch *b, int slot, const double *steady);
 static void batch_after_snapshot(zfv_batch *b, hipEvent_t copied);
+static void member_settle(zfv_handle *h);
 
 static int fail(const std::exception &e) {
   g_last_error = e.what();
@@ -1055,6 +1056,7 @@ int zfv_destroy(zfv_handle *h) {
 
 int zfv_set_state(zfv_handle *h, const double *cvars) {
   ZFV_API_BEGIN
+  member_settle(h);
   to_soa(h, cvars, h->host_stage);
   const size_t bytes = sizeof(double) * h->img.s.nvars * h->img.n;
   HIP_CHECK(hipMemcpyAsync(h->U, h->host_stage, bytes, hipMemcpyHostToDevice, h->stream));
@@ -1065,6 +1067,7 @@ int zfv_set_state(zfv_handle *h, const double *cvars) {
 
 int zfv_get_state(zfv_handle *h, double *cvars) {
   ZFV_API_BEGIN
+  member_settle(h);
   const size_t bytes = sizeof(double) * h->img.s.nvars * h->img.n;
   HIP_CHECK(hipMemcpyAsync(h->host_stage, h->U, bytes, hipMemcpyDeviceToHost, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -1075,6 +1078,7 @@ int zfv_get_state(zfv_handle *h, double *cvars) {
 
 int zfv_snapshot_begin(zfv_handle *h) {
   ZFV_API_BEGIN
+  member_settle(h);
   if (h->snap_pending) throw ZfvError(ZFV_ERR_INVALID, "a snapshot is pending: call zfv_snapshot_end first");
   const size_t count = size_t(h->img.s.nvars) * h->img.n;
   if (!h->snap_dev) {
@@ -1109,6 +1113,7 @@ int zfv_snapshot_end(zfv_handle *h, double *cvars) {
 
 int zfv_set_steady_state(zfv_handle *h, const double *cvars) {
   ZFV_API_BEGIN
+  member_settle(h);
   const int64_t n = h->img.n;
   const int nv = h->img.s.nvars;
   if (!h->steady) h->steady = h->alloc<double>(size_t(nv) * n);
@@ -1125,6 +1130,7 @@ int zfv_set_steady_state(zfv_handle *h, const double *cvars) {
 int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
   (void)t;
   ZFV_API_BEGIN
+  member_settle(h);
   const int64_t n = h->img.n;
   const int nv = h->img.s.nvars;
   to_soa(h, u, h->host_stage);
@@ -1146,6 +1152,7 @@ int zfv_rate(zfv_handle *h, const double *u, double *dudt, double t) {
 int zfv_step(zfv_handle *h, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
+  member_settle(h);
   rec_config(h, h->U);
   enqueue_step(h, dt);
   HIP_CHECK(hipGetLastError());
@@ -1157,6 +1164,7 @@ int zfv_step(zfv_handle *h, double t, double dt) {
 int zfv_steps(zfv_handle *h, int32_t n_steps, double t, double dt) {
   (void)t;
   ZFV_API_BEGIN
+  member_settle(h);
   if (n_steps < 0) throw ZfvError(ZFV_ERR_INVALID, "n_steps < 0");
   rec_config(h, h->U);
   if (!h->graph_exec || h->graph_dt != dt) {
@@ -1182,6 +1190,7 @@ int zfv_advance(zfv_handle *h, double cfl_number, double t0, double t_end, int32
                 double drho_crit_rel, int32_t max_exponent, double *t_final, int32_t *steps_done,
                 int32_t *rejected) {
   ZFV_API_BEGIN
+  member_settle(h);
   check_advance_args(cfl_number, drho_crit_rel, max_exponent);
   if (check_every < 1) check_every = 16;
   rec_config(h, h->U);
@@ -1216,6 +1225,7 @@ int zfv_advance_noalloc(zfv_handle *h, double cfl_number, double t0, double t_en
                         int32_t check_every, double drho_crit_rel, int32_t max_exponent, double *t_final,
                         int32_t *steps_done, int32_t *rejected) {
   ZFV_API_BEGIN
+  member_settle(h);
   check_advance_args(cfl_number, drho_crit_rel, max_exponent);
   if (check_every < 1) check_every = 16;
   int dev = -1;
@@ -1497,6 +1507,7 @@ struct zfv_batch {
   std::vector<zfv_handle *> members;
   hipGraphExec_t graph = nullptr;
   bool dirty = true;
+  bool in_flight = false;  // steps launched (zfv_batch_launch) and not yet waited for
   ~zfv_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (zfv_handle *h : members)
@@ -1530,10 +1541,18 @@ static std::vector<BlockRef> checked_table(const std::vector<int64_t> &counts, i
   return t;
 }
 
+// waits for the batch's stream (and so for steps in flight) and refreshes
+// the host copy of the members' loop state
 static void batch_sync_down(zfv_batch *b) {
   HIP_CHECK(hipMemcpyAsync(b->h_inst.data(), b->d_inst, sizeof(DevInst) * b->max_inst, hipMemcpyDeviceToHost,
                            b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->in_flight = false;
+}
+
+// a member's own calls wait for its batch's steps in flight
+static void member_settle(zfv_handle *h) {
+  if (h->batch && h->batch->in_flight) batch_sync_down(h->batch);
 }
 
 static void batch_upload(zfv_batch *b) {
@@ -1772,8 +1791,9 @@ int zfv_batch_remove(zfv_batch *b, int32_t slot) {
   ZFV_API_END
 }
 
-int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
+int zfv_batch_launch(zfv_batch *b, int32_t n_steps) {
   ZFV_API_BEGIN
+  if (n_steps < 0) throw ZfvError(ZFV_ERR_INVALID, "n_steps must be >= 0");
   if (b->dirty) batch_upload(b);
   if (b->rec_cfg < 0 && batch_has_active(b)) batch_tune(b);
   if (b->rec_cfg < 0) {  // nothing to run on: any configuration will do
@@ -1789,13 +1809,27 @@ int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
     HIP_CHECK(hipGraphDestroy(g));
   }
   for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
-  batch_sync_down(b);
+  b->in_flight = true;
   return ZFV_OK;
   ZFV_API_END
 }
 
+int zfv_batch_wait(zfv_batch *b) {
+  ZFV_API_BEGIN
+  if (b->in_flight) batch_sync_down(b);
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+int zfv_batch_run(zfv_batch *b, int32_t n_steps) {
+  const int rc = zfv_batch_launch(b, n_steps);
+  if (rc != ZFV_OK) return rc;
+  return zfv_batch_wait(b);
+}
+
 int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out) {
   ZFV_API_BEGIN
+  if (b->in_flight) batch_sync_down(b);
   if (b->dirty) batch_upload(b);
   if (rec_threads == 0) {
     if (batch_has_active(b)) batch_tune(b);
@@ -1815,6 +1849,7 @@ int zfv_batch_tune(zfv_batch *b, int32_t rec_threads, zfv_tuning *out) {
 int zfv_batch_query(zfv_batch *b, int32_t slot, double *t, double *dt, int32_t *steps_done, int32_t *active) {
   ZFV_API_BEGIN
   if (slot < 0 || slot >= b->max_inst || !b->members[slot]) throw ZfvError(ZFV_ERR_INVALID, "bad slot");
+  if (b->in_flight) batch_sync_down(b);
   const DevInst &d = b->h_inst[slot];
   if (t) *t = d.t;
   if (dt) *dt = d.dt;

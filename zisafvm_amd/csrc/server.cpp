@@ -214,9 +214,15 @@ void gpu_loop(zfv_server *S) {
       std::lock_guard<std::mutex> lk(S->mu);
       for (auto &kv : S->batches) bs.push_back(kv.second);
     }
+    // every batch's step launched before any is waited for: batches of
+    // different shapes (each on its own stream) overlap on the device
     int run_rc = ZFV_OK;
     for (zfv_batch *b : bs) {
-      int rc = zfv_batch_run(b, 1);
+      int rc = zfv_batch_launch(b, 1);
+      if (rc != ZFV_OK && run_rc == ZFV_OK) run_rc = rc;
+    }
+    for (zfv_batch *b : bs) {
+      int rc = zfv_batch_wait(b);
       if (rc != ZFV_OK && run_rc == ZFV_OK) run_rc = rc;
     }
     // retire finished members; request fields that poll() reads are
