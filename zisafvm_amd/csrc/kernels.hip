@@ -7,7 +7,8 @@
 //                  equilibrium solve (well-balanced), point values on the
 //                  cell's faces (face frame), gravity source
 //   k_face_flux    one thread per interior face: HLLC/Rusanov at the face
-//                  points, integrated once per face
+//                  points, integrated once per face, in the global frame
+//                  times the face's area
 //   k_update       one thread per cell: atomic-free gather of the cell's face
 //                  fluxes, source, flux BC, Butcher-form stage sum, frozen BC
 // FP64 throughout; no MFMA (the per-cell mat-vecs are memory-bound).

@@ -336,12 +336,14 @@ ZFV_HD void face_frame(const double (*w)[3], double *n, double *t1, double *t2, 
 }
 
 // Face frames are time independent: each cell's local faces keep theirs in a
-// table written once at setup (store_face_frames, with face_frame itself), so
-// the cell update reads 3 (2D: t1, area) or 7 (3D: n, t1, area) doubles per
-// face instead of loading the corners and rebuilding frames; n (2D) and t2
-// follow from the stored values with face_frame's own formulas, so every
-// value equals face_frame's.  (The reconstruction, which loads the face
-// vertices for its quadrature points anyway, rebuilds them: measured faster.)
+// table written once at setup (store_face_frames, with face_frame itself),
+// 3 (2D: t1, area) or 7 (3D: n, t1, area) doubles per face, and the interior
+// faces' frames (the left cell's view) are copied into a table in face order
+// for the face kernel (store_face_table); the flux BC of the cell update
+// reads a cell's own.  n (2D) and t2 follow from the stored values with
+// face_frame's own formulas, so every value equals face_frame's.  (The
+// reconstruction, which loads the face vertices for its quadrature points
+// anyway, rebuilds them: measured faster.)
 template <int DIM>
 constexpr int frame_words() { return DIM == 2 ? 3 : 7; }
 // doubles of one cell-face slot of face-point values: Q x NV rounded up to
