@@ -143,12 +143,9 @@ def run_single(z, wl, steps, warmup):
     return s, info, dt, el, t_setup
 
 
-def batched_vs_plain(z, steps):
-    """64 members of the batched configuration: one graph per step for all of
-    them, versus the same members one after another on the plain path."""
-    from zisafvm_amd import workloads as W
-
-    wls = W.batched_2d()
+def batched_vs_plain(z, steps, wls):
+    """Members of a batched workload: one graph per step for all of them,
+    versus the same members one after another on the plain path."""
     cells = sum(w.mesh.n_cells for w in wls)
     try:
         b = z.Batch(wls[0].params, cell_capacity=cells, face_capacity=2 * cells, max_instances=len(wls))
@@ -249,6 +246,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", action="store_true",
                     help="add the batched-vs-plain leg (64 members)")
+    ap.add_argument("--batch-small", action="store_true",
+                    help="add a batched-vs-plain leg of 8 small members (1,000-2,000 triangles)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--serve", action="store_true", help="add the serving leg (Poisson request streams)")
     ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates (requests/s)")
@@ -380,10 +379,21 @@ def main():
         "measured_copy_gbs": z.bandwidth_probe(1 << 30, 10),
     }
     s.close()
+    if args.batch or args.batch_small:
+        from zisafvm_amd import workloads as W
     if args.batch:
         log("batched vs plain ...")
-        out["batched_vs_plain"] = batched_vs_plain(z, args.batch_steps)
+        out["batched_vs_plain"] = batched_vs_plain(z, args.batch_steps, W.batched_2d())
         log(json.dumps(out["batched_vs_plain"]))
+    if args.batch_small:
+        # the batched configuration's mix (vortices and atmospheres, order 3)
+        # at the member sizes the batch tests run: 8 members of 1,000-2,000
+        # triangles, where per-member launches dominate the plain path
+        log("batched vs plain, small members ...")
+        wls = W.batched_2d(n_instances=8, min_cells=1000, max_cells=2000)
+        out["batched_small"] = batched_vs_plain(z, args.batch_steps, wls)
+        out["batched_small"]["workload"] = "8 members of 1,000-2,000 triangles (not the 64-member configuration)"
+        log(json.dumps(out["batched_small"]))
     if args.serve:
         log("serving ...")
         rates = [float(r) for r in args.serve_rates.split(",") if r]
