@@ -830,6 +830,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // local equilibrium of this cell: every lane solves it (identical results)
   double th[2] = {0.0, 0.0};
   double phi_ref = 0.0;
+  double ar1 = -1.0;  // isothermal: the average of exp(a (phi - phi_ref)) over the cell, once solved
   const bool wb = m.eq != 0;
   bool eq_found = false;
   if (wb) {
@@ -876,6 +877,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       double t1[2] = {1.0, RT};
       double ar, aE;
       cell_avg(t1, ar, aE);
+      ar1 = ar;
       th[0] = rho0 / ar;
       th[1] = RT;
       eq_found = true;
@@ -1017,7 +1019,17 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   double q0;
   {
     double er = 0.0, eE = 0.0;
-    if (eq_sub) own_eq(er, eE);
+    if (eq_sub) {
+      // isothermal: the equilibrium is th0 times the function the solve just
+      // averaged over this cell (same points, weights and order), so its own
+      // average is th0 ar1 (to rounding), with E = b rho pointwise
+      if (ar1 > 0.0) {
+        er = th[0] * ar1;
+        eE = er * thc.b;
+      } else {
+        own_eq(er, eE);
+      }
+    }
     if (m.eqc != nullptr && !reuse && g == 0) {
       // the recomputed equilibrium's own average (zero if none), which the
       // next calls measure the threshold against
