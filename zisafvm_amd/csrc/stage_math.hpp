@@ -789,6 +789,16 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   double sc_rho = rho0, sc_E = Eint0, sc_a = sqrt(gamma * (Eint0 * (gamma - 1.0)) / rho0);
   if (m.scaling == 1) sc_rho = sc_E = sc_a = 1.0;
   const bool is_rho = g == 0, is_E = g == NV - 1;
+  // The first two chunks' stencil row entries (and in 3D the first chunk's
+  // member states) are loaded before the equilibrium solve, whose arithmetic
+  // then covers their latency (every row entry names a cell, first-order
+  // cells included)
+  constexpr int R_ALL = RC + NB * RB;
+  const int32_t rw_pre0 = g < R_ALL ? m.rows[g * n + i] : int32_t(0);
+  const int32_t rw_pre1 = NV + g < R_ALL ? m.rows[(NV + g) * n + i] : int32_t(0);
+  double uk_pre[NV];
+  if (DIM == 3)
+    for (int k = 0; k < NV; ++k) uk_pre[k] = U[g * n + (xch.bcast_i(rw_pre0, k) & ROW_CELL_MASK)];
 
   double vtx[DIM + 1][3];
   load_corners<DIM>(m, i, vtx);
@@ -1131,8 +1141,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // the rest (the last central rows and the one-sided ones) in a second, so
     // the one-sided coefficients b hold registers only there.
     constexpr int R = RC + NB * RB;
-    int32_t rw_cur = g < R ? m.rows[g * n + i] : int32_t(0);
-    int32_t rw_nxt = NV + g < R ? m.rows[(NV + g) * n + i] : int32_t(0);
+    int32_t rw_cur = rw_pre0;
+    int32_t rw_nxt = rw_pre1;
     Member mb_cur{};
     if (eq_sub && g < R) load_member(rw_cur, mb_cur);
     // stages chunk r0: loads the chunk's member states, advances the
@@ -1144,7 +1154,8 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       const int my_r = r0 + g;
       const bool mine = my_r < R;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) uk[k] = U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
+      for (int k = 0; k < NV; ++k)
+        uk[k] = (DIM == 3 && r0 == 0) ? uk_pre[k] : U[g * n + (xch.bcast_i(rw_cur, k) & ROW_CELL_MASK)];
       // a one-sided row whose member is a central row of an earlier chunk
       // takes that row's averages from LDS (the same values: same member,
       // same function) instead of loading the member and averaging again;
