@@ -17,6 +17,14 @@ counted).  The JSON line also carries
     / its average HIP-event duration on the handle's stream;
   - cpu_baseline: the oracle (the reference's OpenMP CPU path restated) timed
     on all of this host's cores over a bounded sample, and on one thread.
+Small legs, run by default, each in a child process (its own HIP context;
+a failure there is reported in its entry and leaves this run's line intact),
+at the member sizes the batch and serving tests run:
+  batched_small  batched_vs_plain on 8 members of the batched configuration's
+                 mix at 1,000-2,000 triangles (--no-small skips both);
+  serving_small  Poisson streams of 16 small requests (2D 1,000-2,000
+                 triangles, 3D 3,200-5,000 tetrahedra) through the serving
+                 runtime: p50/p99 latency.
 Opt-in legs (the library refuses batches above 65,536 cells or 8 members,
 so at the configurations' sizes both legs report that refusal):
   --batch  batched_vs_plain: the batched configuration (64 members, 5k-200k
@@ -235,6 +243,48 @@ def serving(z, rates, n_requests, budget_gib):
     }
 
 
+def serving_small(z, rates):
+    """Poisson streams of small requests (the serving tests' sizes) through
+    the serving runtime: p50/p99 latency per arrival rate."""
+    from zisafvm_amd import serving as S
+    from zisafvm_amd import workloads as W
+
+    reqs = W.serving_requests(16, min_tris=1000, max_tris=2000, min_tets=3200, max_tets=5000, t_min=0.005,
+                              t_max=0.05)
+    est = [S.estimate_device_bytes(w.mesh, w.params) for w in reqs]
+    runs = []
+    for rate in rates:
+        srv = S.Server(4 * max(est), batch_cell_capacity=1 << 16, max_members=8, setup_workers=2)
+        res = S.poisson_run(srv, reqs, rate, seed=int(rate * 7))
+        srv.close()
+        runs.append(res.summary())
+    return {
+        "requests": len(reqs),
+        "mix": "every 4th a 3D polytrope (3,200-5,000 tets), others 2D vortex / isothermal (1,000-2,000 "
+               "triangles), t_end U[0.005, 0.05], CWENO-AO 3 (not the serving configuration's sizes)",
+        "cells": [int(w.mesh.n_cells) for w in reqs],
+        "hbm_budget_bytes": int(4 * max(est)),
+        "latency_definition": "submit to result, including host setup of the instance",
+        "runs": runs,
+    }
+
+
+def child_leg(name, timeout_s=240):
+    """Runs one small leg in a child process and returns its JSON (or what
+    went wrong); this process's own results are unaffected either way."""
+    import subprocess
+
+    try:
+        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--leg", name], capture_output=True,
+                           text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s} s"}
+    for line in reversed(r.stdout.splitlines()):
+        if line.startswith("{"):
+            return json.loads(line)
+    return {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-800:]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +303,8 @@ def main():
     ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=1.0)
+    ap.add_argument("--no-small", action="store_true", help="skip the small batched and serving legs")
+    ap.add_argument("--leg", default="", help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the streaming-copy calibration kernel (1 GiB x 5) and exit")
     args = ap.parse_args()
@@ -262,6 +314,17 @@ def main():
         return 2
 
     import zisafvm_amd as z
+
+    if args.leg == "batch-small":
+        from zisafvm_amd import workloads as W
+
+        res = batched_vs_plain(z, 200, W.batched_2d(n_instances=8, min_cells=1000, max_cells=2000))
+        res["workload"] = "8 members of 1,000-2,000 triangles (not the 64-member configuration)"
+        print(json.dumps(res), flush=True)
+        return 0
+    if args.leg == "serve-small":
+        print(json.dumps(serving_small(z, [50.0, 200.0, 1000.0])), flush=True)
+        return 0
 
     if args.copy_probe_only:
         bw = z.bandwidth_probe(1 << 30, 5)
@@ -379,6 +442,12 @@ def main():
         "measured_copy_gbs": z.bandwidth_probe(1 << 30, 10),
     }
     s.close()
+    if not args.no_small:
+        log("small batched and serving legs (child processes) ...")
+        out["batched_small"] = child_leg("batch-small")
+        log(json.dumps(out["batched_small"]))
+        out["serving_small"] = child_leg("serve-small")
+        log(json.dumps(out["serving_small"]))
     if args.batch or args.batch_small:
         from zisafvm_amd import workloads as W
     if args.batch:

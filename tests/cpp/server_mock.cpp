@@ -69,6 +69,18 @@ int api_fail(const std::exception &e) {
 extern "C" {
 
 const char *zfv_last_error(void) { return g_err.c_str(); }
+int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out);
+}
+
+// zfv_create's two halves: the mock builds the whole handle in the host half
+namespace zfv {
+int create_host(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out) {
+  return zfv_create(mesh, params, out);
+}
+int create_device(zfv_handle *) { return ZFV_OK; }
+}  // namespace zfv
+
+extern "C" {
 
 int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out) {
   (void)params;

@@ -10,6 +10,13 @@
 
 namespace zfv {
 
+// zfv_create in two halves (kernels.hip), for the serving runtime's setup
+// workers: create_host builds the setup image with no device call,
+// create_device allocates, uploads and runs the setup kernels; on failure
+// the caller deletes the handle with zfv_destroy
+int create_host(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out);
+int create_device(zfv_handle *h);
+
 struct ZfvError : std::runtime_error {
   int code;
   ZfvError(int code, const std::string &msg) : std::runtime_error(msg), code(code) {}

@@ -891,12 +891,43 @@ extern "C" {
 const char *zfv_last_error(void) { return g_last_error.c_str(); }
 
 int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out) {
+  if (!out) {
+    g_last_error = "null argument";
+    return ZFV_ERR_INVALID;
+  }
+  zfv_handle *h = nullptr;
+  int rc = zfv::create_host(mesh, params, &h);
+  if (rc == ZFV_OK) rc = zfv::create_device(h);
+  if (rc != ZFV_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return ZFV_OK;
+}
+
+}  // extern "C"
+
+namespace zfv {
+
+// the host half of zfv_create: the setup image (stencils, pseudo-inverses,
+// tables), no device call
+int create_host(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out) {
   ZFV_API_BEGIN
   if (!mesh || !params || !out) throw ZfvError(ZFV_ERR_INVALID, "null argument");
-  check_device();
   auto h = std::make_unique<zfv_handle>();
   h->params = *params;
   build_image(*mesh, *params, h->img);
+  *out = h.release();
+  return ZFV_OK;
+  ZFV_API_END
+}
+
+// the device half: stream, allocations, uploads and the setup kernels
+int create_device(zfv_handle *h) {
+  ZFV_API_BEGIN
+  check_device();
+  const zfv_params *params = &h->params;
   const HostImage &im = h->img;
   const Shape &s = im.s;
   h->L = pick_launcher(s);
@@ -1035,10 +1066,13 @@ int zfv_create(const zfv_mesh *mesh, const zfv_params *params, zfv_handle **out)
     HIP_CHECK(hipMemset(m.eqcnt, 0, sizeof(int32_t) * size_t(n)));
   }
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h->host_stage), sizeof(double) * size_t(nv) * n));
-  *out = h.release();
   return ZFV_OK;
   ZFV_API_END
 }
+
+}  // namespace zfv
+
+extern "C" {
 
 int zfv_destroy(zfv_handle *h) {
   if (!h) return ZFV_OK;

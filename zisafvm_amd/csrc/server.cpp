@@ -1,11 +1,13 @@
 // Serving runtime: continuous batching of solve requests on one GPU.
 //
-// Not in the reference (it runs one experiment per process).  Built only on
-// the library's public C-ABI (include/zfv.h):
+// Not in the reference (it runs one experiment per process).  Built on the
+// library's public C-ABI (include/zfv.h) and the two halves of zfv_create
+// (common.hpp):
 //   submit   -> pending queue (request copied)
 //   admit    -> when the request's estimated HBM fits the budget, a setup
-//               worker builds its instance (zfv_create: geometry, stencils,
-//               pseudo-inverses, upload) while the GPU keeps stepping others
+//               worker builds its instance (zfv_create's two halves: the
+//               host setup image, then allocation and upload) while the GPU
+//               keeps stepping others
 //   join     -> at the next step boundary the instance joins the batch of its
 //               kernel shape (zfv_batch_add); every batch advances one SSP3
 //               step per loop iteration with each member's own CFL dt
@@ -72,6 +74,11 @@ struct zfv_server {
   int64_t batch_cells = 0;
   int32_t max_members = 0;
   std::mutex mu;
+  // held by a setup worker around its instance calls (allocation, uploads,
+  // setup kernels) and by the GPU loop around its batch launches, which may
+  // capture a step graph: no other thread's device calls run while a graph
+  // is being captured
+  std::mutex hip_mu;
   std::condition_variable cv;
   std::deque<std::shared_ptr<zfv::Request>> pending;  // waiting for HBM
   std::deque<std::shared_ptr<zfv::Request>> setup_q;  // admitted, waiting for a worker
@@ -123,15 +130,24 @@ void setup_worker(zfv_server *S) {
     zfv_mesh m{r->mesh.n_dims, r->mesh.n_cells, r->mesh.vertex_ids.data(), r->mesh.corners.data(),
                r->mesh.ghost.data()};
     zfv_handle *h = nullptr;
-    int rc = zfv_create(&m, &r->params, &h);
-    if (rc == ZFV_OK) rc = zfv_set_state(h, r->u0.data());
-    if (rc == ZFV_OK && !r->steady.empty()) rc = zfv_set_steady_state(h, r->steady.data());
+    std::string err;
+    int rc = zfv::create_host(&m, &r->params, &h);
+    {
+      std::lock_guard<std::mutex> hl(S->hip_mu);
+      if (rc == ZFV_OK) rc = zfv::create_device(h);
+      if (rc == ZFV_OK) rc = zfv_set_state(h, r->u0.data());
+      if (rc == ZFV_OK && !r->steady.empty()) rc = zfv_set_steady_state(h, r->steady.data());
+      if (rc != ZFV_OK) {
+        err = zfv_last_error();
+        if (h) zfv_destroy(h);
+        h = nullptr;
+      }
+    }
     std::lock_guard<std::mutex> lk(S->mu);
     S->building -= 1;
     if (rc != ZFV_OK) {
       r->state = ZFV_REQ_FAILED;
-      r->error = zfv_last_error();
-      if (h) zfv_destroy(h);
+      r->error = err;
       S->reserved -= r->est_bytes;
       r->t_done = clock_t_::now();
       admit_locked(S);
@@ -217,9 +233,12 @@ void gpu_loop(zfv_server *S) {
     // every batch's step launched before any is waited for: batches of
     // different shapes (each on its own stream) overlap on the device
     int run_rc = ZFV_OK;
-    for (zfv_batch *b : bs) {
-      int rc = zfv_batch_launch(b, 1);
-      if (rc != ZFV_OK && run_rc == ZFV_OK) run_rc = rc;
+    {
+      std::lock_guard<std::mutex> hl(S->hip_mu);
+      for (zfv_batch *b : bs) {
+        int rc = zfv_batch_launch(b, 1);
+        if (rc != ZFV_OK && run_rc == ZFV_OK) run_rc = rc;
+      }
     }
     for (zfv_batch *b : bs) {
       int rc = zfv_batch_wait(b);

@@ -184,6 +184,10 @@ def poisson_run(server: Server, requests, rate: float, seed: int = 0, timeout_s:
     for r in rids:
         st = server.poll(r)
         if st.state != DONE:
+            try:
+                server.result(r, st.n_cells)  # raises with the request's error
+            except Exception as e:
+                raise RuntimeError(f"request {r} failed: {e}") from None
             raise RuntimeError(f"request {r} failed")
         lat.append(st.latency_s)
         cs += st.n_cells * 3 * st.steps
