@@ -450,8 +450,8 @@ void batch_step(EmuBatch &B, int waves, bool xcd, int rec_sample) {
   const auto tf = block_table(faces, 256, big, xcd);
   const auto tr = block_table(cells, int64_t(waves) * GPW, big, xcd);
   DevInst *insts = B.inst.data();
-  for (int s = 0; s < n_inst; ++s) kb_begin_instance(insts[s]);
-  // CFL: per block the minimum of its threads' candidates, then the minimum
+  // CFL (the minimum starts at 1e300: set when a member joins and at every
+  // step's end): per block the minimum of its threads' candidates, then the minimum
   // over blocks per instance (the kernel's atomic min on the bits of
   // positive doubles)
   for (const BlockRef &b : tc) {
@@ -462,7 +462,6 @@ void batch_step(EmuBatch &B, int waves, bool xcd, int rec_sample) {
     std::memcpy(&bits, &r, sizeof(bits));
     insts[b.inst].dt_bits = std::min(insts[b.inst].dt_bits, bits);
   }
-  for (int s = 0; s < n_inst; ++s) kb_choose_instance(insts[s]);
   for (int stage = 0; stage < 3; ++stage) {
     // reconstruction: the NV lanes of a group run as host threads in lockstep
     LaneBarrier bar(NV);
@@ -575,6 +574,7 @@ int zfv_emu_batch_create(void **members, int n, const double *const *u0, const d
       d.cfl = cfl[k];
       d.t = 0.0;
       d.t_end = t_end[k];
+      d.dt_bits = kb_no_bound();
       d.active = 1;
       d.max_steps = max_steps[k] > 0 ? max_steps[k] : 0x7fffffff;
       B->inst[k] = d;

@@ -33,6 +33,18 @@ struct BlockRef {
   int local;  // block index inside the instance
 };
 
+// the step's dt of a member: cfl times its CFL minimum (the bits of a
+// positive double in dt_bits), clamped to its end time
+ZFV_HD double kb_step_dt(const DevInst &d) {
+  double m;
+  const unsigned long long bits = d.dt_bits;
+  memcpy(&m, &bits, sizeof(m));
+  double dt = d.cfl * m;
+  const double rest = d.t_end - d.t;
+  if (dt > rest) dt = rest;
+  return dt;
+}
+
 // groups of NV lanes per 64-wide wavefront (2D: 16 cells, 3D: 12 cells)
 template <int NV>
 constexpr int groups_per_wave() {
@@ -76,6 +88,7 @@ ZFV_HD void kb_reconstruct_group(const DevInst *insts, BlockRef b, int waves, in
   if (b.inst < 0) return;
   const DevInst &d = insts[b.inst];
   if (!d.active) return;
+  if (stage == 0 && !(kb_step_dt(d) > 0.0)) return;  // stops this step (kb_update_thread)
   const int64_t i = (int64_t(b.local) * waves + wave) * GPW + grp;
   if (grp >= GPW || i >= d.m.n) return;
   if (d.m.steady != nullptr && (d.m.meta[i] & META_FAR_GHOST) == META_FAR_GHOST) return;
@@ -97,11 +110,24 @@ ZFV_HD void kb_face_flux_thread(const DevInst *insts, BlockRef b, int t) {
 // K[stage] = L(U_stage), U_next = U0 + dt sum_j c_j K[j]
 // (src/zisa/ode/runge_kutta.cpp:122-165), then the frozen BC
 template <int DIM>
-ZFV_HD void kb_update_thread(const DevInst *insts, BlockRef b, int t, int stage) {
+ZFV_HD void kb_update_thread(DevInst *insts, BlockRef b, int t, int stage) {
   constexpr int NV = DIM + 2;
   if (b.inst < 0) return;
-  const DevInst &d = insts[b.inst];
+  DevInst &d = insts[b.inst];
   if (!d.active) return;
+  double dt = d.dt;
+  if (stage == 0) {
+    // the step's dt, from the CFL minimum of this step: every thread of the
+    // member forms the same value; the member's first thread records it for
+    // the later stages and the step's end (and stops a member that cannot
+    // advance, whose threads all leave here)
+    dt = kb_step_dt(d);
+    if (b.local == 0 && t == 0) {
+      d.dt = dt;
+      if (!(dt > 0.0)) d.active = 0;
+    }
+    if (!(dt > 0.0)) return;
+  }
   const int64_t i = int64_t(b.local) * 256 + t;
   const Img &m = d.m;
   const int64_t n = m.n;
@@ -135,7 +161,7 @@ ZFV_HD void kb_update_thread(const DevInst *insts, BlockRef b, int t, int stage)
       dudt += c1 * d.K[(int64_t(1) * NV + v) * n + i];
       dudt += c2 * L[v];
     }
-    double un = d.U[v * n + i] + d.dt * dudt;
+    double un = d.U[v * n + i] + dt * dudt;
     if (ghost) un = m.steady[v * n + i];
     Un[v * n + i] = un;
   }
@@ -165,9 +191,10 @@ ZFV_HD double kb_cfl_thread(const DevInst &d, BlockRef b, int t) {
   return r;
 }
 
-// per-instance step bookkeeping: start of a step (the CFL minimum starts at
-// 1e300), the dt of the step (cfl times the minimum, clamped to t_end; an
-// instance that cannot advance stops), end of a step
+// per-instance step bookkeeping: the CFL minimum of a step starts at 1e300
+// (set when the member joins and at the end of every step); the step's dt is
+// cfl times the minimum, clamped to t_end (an instance that cannot advance
+// stops: kb_update_thread of stage 0); the end of a step advances the time
 ZFV_HD unsigned long long kb_no_bound() {
   const double big = 1.0e300;
   unsigned long long bits;
@@ -175,25 +202,12 @@ ZFV_HD unsigned long long kb_no_bound() {
   return bits;
 }
 
-ZFV_HD void kb_begin_instance(DevInst &d) { d.dt_bits = kb_no_bound(); }
-
-ZFV_HD void kb_choose_instance(DevInst &d) {
-  if (!d.active) return;
-  double m;
-  unsigned long long bits = d.dt_bits;
-  memcpy(&m, &bits, sizeof(m));
-  double dt = d.cfl * m;
-  const double rest = d.t_end - d.t;
-  if (dt > rest) dt = rest;
-  d.dt = dt;
-  if (!(dt > 0.0)) d.active = 0;
-}
-
 ZFV_HD void kb_end_instance(DevInst &d) {
   if (!d.active) return;
   d.t += d.dt;
   d.steps_done += 1;
   if (d.t >= d.t_end || d.steps_done >= d.max_steps) d.active = 0;
+  d.dt_bits = kb_no_bound();
 }
 
 }  // namespace zfv

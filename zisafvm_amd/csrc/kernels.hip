@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void kb_face_flux(const DevInst *insts, const 
 }
 
 template <int DIM>
-__global__ __launch_bounds__(256) void kb_update(const DevInst *insts, const BlockRef *blk, int stage) {
+__global__ __launch_bounds__(256) void kb_update(DevInst *insts, const BlockRef *blk, int stage) {
   kb_update_thread<DIM>(insts, blk[blockIdx.x], threadIdx.x, stage);
 }
 
@@ -336,16 +336,6 @@ __global__ __launch_bounds__(256) void kb_cfl(DevInst *insts, const BlockRef *bl
   }
 }
 
-// per-instance step bookkeeping (one thread per instance)
-__global__ void kb_begin_step(DevInst *insts, int n_inst) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n_inst) kb_begin_instance(insts[k]);
-}
-
-__global__ void kb_choose_dt(DevInst *insts, int n_inst) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n_inst) kb_choose_instance(insts[k]);
-}
 
 // A single instance's time loop on the device (zfv_advance): TimeLoop's
 // step with LocalCFL (src/zisa/fvm_loops/time_loop.cpp:36-81,
@@ -1692,9 +1682,9 @@ static bool batch_has_active(const zfv_batch *b) {
 static void batch_enqueue_step(zfv_batch *b) {
   const unsigned nc = unsigned(b->n_cell_blocks), nf = unsigned(b->n_face_blocks), nr = unsigned(b->n_rec_blocks);
   const unsigned ib = unsigned((b->max_inst + 63) / 64);
-  hipLaunchKernelGGL(kb_begin_step, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
+  // the step's CFL minimum (reset at the end of the previous step or when
+  // the member joined); stage 0's update turns it into the step's dt
   b->L.bcfl(b->d_inst, b->d_bcell, nc, b->stream);
-  hipLaunchKernelGGL(kb_choose_dt, dim3(ib), dim3(64), 0, b->stream, b->d_inst, b->max_inst);
   for (int s = 0; s < 3; ++s) {
     b->L.brec[b->ph](b->d_inst, b->d_brec, nr, s, b->rec_cfg, b->stream);
     b->L.bface(b->d_inst, b->d_bface, nf, b->stream);
@@ -1798,6 +1788,7 @@ int zfv_batch_add(zfv_batch *b, zfv_handle *h, double cfl, double t0, double t_e
   d.t = t0;
   d.t_end = t_end;
   d.dt = 0.0;
+  d.dt_bits = kb_no_bound();
   d.active = 1;
   d.steps_done = 0;
   d.max_steps = max_steps > 0 ? max_steps : 0x7fffffff;
