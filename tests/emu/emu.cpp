@@ -181,6 +181,7 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
     for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
     m.mode = p->reconstruction;
     m.scaling = p->scaling;
+    m.vol_mean = volume_mean_is_constant_term(*p, im.s) ? 1 : 0;
     m.flux = p->flux;
     m.eq = p->equilibrium;
     m.gravity = p->gravity;

@@ -78,6 +78,19 @@ def test_unity_scaling(case):
     assert max(rel_err(L_euler, Lo)) > 10 * TOL
 
 
+@pytest.mark.parametrize("volume_deg", [4, 2])
+def test_isothermal_volume_source_both_forms(volume_deg):
+    # the light set's gravity volume term: from the reconstruction's constant
+    # coefficient when the volume rule integrates the cubic basis exactly
+    # (degree 4), over the rule's points when it does not (degree 2)
+    wl = W.isothermal_2d(n=14, amplitude=1e-3)
+    p = wl.params
+    p.volume_deg = volume_deg
+    o, e = Oracle(wl.mesh, p), Emu(wl.mesh, p)
+    errs = wb_err(e.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, p.gamma)
+    assert max(errs) <= TOL, errs
+
+
 def test_polytrope_3d():
     wl = W.polytrope_3d(n=12, amplitude=1e-3)
     o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)

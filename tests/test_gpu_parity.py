@@ -128,6 +128,21 @@ def test_unity_scaling_rate(gpu, case):
     assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("volume_deg", [4, 2])
+def test_isothermal_volume_source_both_forms_rate(gpu, volume_deg):
+    # the gravity volume term from the constant coefficient (degree-4 rule)
+    # and over the rule's points (degree 2, which does not integrate cubics)
+    from conftest import wb_err
+
+    wl = W.isothermal_2d(n=16, amplitude=1e-3)
+    p = wl.params
+    p.volume_deg = volume_deg
+    s, o = z.Solver(wl.mesh, p), Oracle(wl.mesh, p)
+    errs = wb_err(s.rate(wl.u0), o.rate(wl.u0), wl.mesh, wl.u0, p.gamma)
+    print("isothermal volume_deg", volume_deg, "error per variable", errs)
+    assert max(errs) <= TOL
+
+
 def test_polytrope_3d_rate(gpu):
     from conftest import wb_err
 

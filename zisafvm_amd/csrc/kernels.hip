@@ -950,6 +950,7 @@ int create_device(zfv_handle *h) {
   for (int k = 0; k < 5; ++k) m.lw[k] = im.linear_weights[k];
   m.mode = params->reconstruction;
   m.scaling = params->scaling;
+  m.vol_mean = volume_mean_is_constant_term(*params, s) ? 1 : 0;
   m.flux = params->flux;
   m.eq = params->equilibrium;
   m.gravity = params->gravity;

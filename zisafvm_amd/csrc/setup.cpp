@@ -818,6 +818,13 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
 }
 
 
+bool volume_mean_is_constant_term(const zfv_params &p, const Shape &s) {
+  int degc = 0;
+  while (dof(degc + 1, s.dim) <= s.dofc) ++degc;
+  const int mdeg = p.moments_deg == -1 ? (s.dim == 2 ? 5 : 3) : p.moments_deg;
+  return p.volume_deg >= degc && mdeg >= degc;
+}
+
 Shape shape_for(const zfv_params &p) {
   Shape s;
   s.dim = p.n_dims;

@@ -90,6 +90,11 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img);
 // Shape implied by the parameters (throws if the GPU kernels have no
 // instantiation for it).
 Shape shape_for(const zfv_params &p);
+// true when the volume rule and the cell moments both integrate the central
+// polynomial's basis exactly: then the quadrature mean of a reconstruction
+// over its cell is its constant coefficient (every other basis function has
+// mean zero), which the light set's gravity volume term uses
+bool volume_mean_is_constant_term(const zfv_params &p, const Shape &s);
 
 // The caller's n x 5 row-major state (AllVariables::cvars) to and from the
 // product's nvars x n planes in product order (perm: product index -> caller

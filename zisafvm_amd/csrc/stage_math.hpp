@@ -83,6 +83,7 @@ struct Img {
   double lw[5];  // normalized linear weights
   int mode;      // 0 CWENO-AO, 1 WENO-AO
   int scaling;   // 0 EulerScaling, 1 UnityScaling
+  int vol_mean;  // 1: a reconstruction's volume-rule mean over its cell is its constant coefficient
   int flux;      // 0 HLLC, 1 Rusanov
   int eq;        // 0 none, 1 isentropic, 2 isothermal
   int gravity;   // 0 none, 1 axial, 2 radial, 3 polytrope
@@ -1433,9 +1434,19 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
     // delta(x_q) d_d phi(x_q) for every d; the density lane's sums feed the
     // momentum lanes, momentum lane d's d-th sum the energy lane.
     double acc[DIM];
-    if (PH == PH_LIGHT) {
-      // constant gradient (axial gravity): the quadrature over the points,
-      // cheaper than reading the source moments
+    if (PH == PH_LIGHT && m.vol_mean) {
+      // constant gradient (axial gravity): sum_q w_q delta(x_q) grad phi is
+      // vol grad phi times the rule's mean of delta over the cell, which is
+      // its constant coefficient (the rule and the moments integrate every
+      // other basis function, X^alpha - c_alpha, to zero)
+      double gp[3];
+      grad_phi<DIM, PH>(m, xc, gp);
+      const double s0 = vol_i * (scale_g * a[0]);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) acc[d] = -s0 * gp[d];
+    } else if (PH == PH_LIGHT) {
+      // constant gradient, rules that do not integrate the basis exactly:
+      // the quadrature over the points
       load_corners<DIM>(m, i, vtx);
 #pragma unroll 1
       for (int q = 0; q < m.qv; ++q) {
