@@ -91,6 +91,25 @@ def test_isothermal_volume_source_both_forms(volume_deg):
     assert max(errs) <= TOL, errs
 
 
+def _isothermal_3d(n=8, amplitude=1e-3):
+    # 3D light set: isothermal atmosphere under axial gravity in the ball
+    # (the workloads' atmosphere, phi = y), HLLC, order 3
+    from zisafvm_amd.workloads import ball_mesh_3d
+
+    mesh = ball_mesh_3d(n, radius=1.0, ghost_radius=0.85, jitter=0.1, seed=7)
+    p = z.make_params(n_dims=3, order=3, equilibrium=z.ZFV_EQ_ISOTHERMAL, gravity=z.ZFV_GRAVITY_AXIAL,
+                      gravity_g=1.0)
+    u0 = W.cell_average(mesh, lambda x: W.isothermal_atmosphere(x, amplitude=amplitude))
+    return mesh, p, u0
+
+
+def test_isothermal_3d_light_set():
+    mesh, p, u0 = _isothermal_3d()
+    o, e = Oracle(mesh, p), Emu(mesh, p)
+    errs = wb_err(e.rate(u0), o.rate(u0), mesh, u0, p.gamma)
+    assert max(errs) <= TOL, errs
+
+
 def test_polytrope_3d():
     wl = W.polytrope_3d(n=12, amplitude=1e-3)
     o, e = Oracle(wl.mesh, wl.params), Emu(wl.mesh, wl.params)

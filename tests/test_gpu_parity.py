@@ -143,6 +143,21 @@ def test_isothermal_volume_source_both_forms_rate(gpu, volume_deg):
     assert max(errs) <= TOL
 
 
+def test_isothermal_3d_light_set_rate(gpu):
+    # the 3D light-set kernel (isothermal atmosphere, axial gravity, HLLC)
+    from conftest import wb_err
+    from zisafvm_amd.workloads import ball_mesh_3d
+
+    mesh = ball_mesh_3d(8, radius=1.0, ghost_radius=0.85, jitter=0.1, seed=7)
+    p = z.make_params(n_dims=3, order=3, equilibrium=z.ZFV_EQ_ISOTHERMAL, gravity=z.ZFV_GRAVITY_AXIAL,
+                      gravity_g=1.0)
+    u0 = W.cell_average(mesh, lambda x: W.isothermal_atmosphere(x, amplitude=1e-3))
+    s, o = z.Solver(mesh, p), Oracle(mesh, p)
+    errs = wb_err(s.rate(u0), o.rate(u0), mesh, u0, p.gamma)
+    print("isothermal 3D light set error per variable", errs)
+    assert max(errs) <= TOL
+
+
 def test_polytrope_3d_rate(gpu):
     from conftest import wb_err
 
