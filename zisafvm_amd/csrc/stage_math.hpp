@@ -63,7 +63,8 @@ struct Img {
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
   const double *ffr;     // per interior face its frame and area, t1, area (2D) or n, t1, area (3D), x ne
   // gravity: per cell vol sum_q vw_q b_alpha(x_q) d_d phi(x_q) over its volume
-  // points, (alpha DIM + d) x n (source_moments), or null
+  // points, n x (alpha DIM + d), each cell's DOFC x DIM values contiguous
+  // (store_source_moments), or null
   const double *smom;
   // the potential at every cell's corners ((d + 1) x n; the light set with an
   // equilibrium, whose axial potential is affine), or null
@@ -646,7 +647,7 @@ ZFV_HD void store_source_moments(const Img &m, int64_t i, double *tab) {
       for (int d = 0; d < DIM; ++d) acc[al][d] = (q == 0) ? w * (b[al] * gp[d]) : fma(w, b[al] * gp[d], acc[al][d]);
   }
   for (int al = 0; al < DOFC; ++al)
-    for (int d = 0; d < DIM; ++d) tab[(int64_t(al) * DIM + d) * n + i] = acc[al][d];
+    for (int d = 0; d < DIM; ++d) tab[int64_t(i) * (DOFC * DIM) + al * DIM + d] = acc[al][d];
 }
 
 // ---- ideal gas / fluxes -------------------------------------------------------
@@ -1464,10 +1465,21 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       // from the polynomial and the cell's source moments (store_source_moments)
 #pragma unroll
       for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+      // (the cell's DOFC x DIM moments are contiguous: 16-byte loads)
+      double sm[DOFC * DIM];
+      {
+        const PPair *sp = reinterpret_cast<const PPair *>(m.smom + int64_t(i) * (DOFC * DIM));
+#pragma unroll
+        for (int k = 0; k < DOFC * DIM / 2; ++k) {
+          const PPair v = sp[k];
+          sm[2 * k] = v.lo;
+          sm[2 * k + 1] = v.hi;
+        }
+      }
 #pragma unroll
       for (int al = 0; al < DOFC; ++al)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) acc[d] = fma(a[al], m.smom[(int64_t(al) * DIM + d) * n + i], acc[d]);
+        for (int d = 0; d < DIM; ++d) acc[d] = fma(a[al], sm[al * DIM + d], acc[d]);
 #pragma unroll
       for (int d = 0; d < DIM; ++d) acc[d] = -scale_g * acc[d];
     }
