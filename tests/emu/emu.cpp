@@ -216,9 +216,9 @@ int zfv_emu_create(const zfv_mesh *mesh, const zfv_params *p, void **out) {
       e->phim.assign(size_t(2) * im.n, 0.0);
       for (int64_t i = 0; i < im.n; ++i) {
         if (im.s.dim == 2)
-          phi_moments<2>(m, i, e->phim[i], e->phim[im.n + i]);
+          phi_moments<2>(m, i, e->phim[2 * i], e->phim[2 * i + 1]);
         else
-          phi_moments<3>(m, i, e->phim[i], e->phim[im.n + i]);
+          phi_moments<3>(m, i, e->phim[2 * i], e->phim[2 * i + 1]);
       }
       m.phim = e->phim.data();
     }

@@ -271,8 +271,8 @@ __global__ __launch_bounds__(256) void k_phi_moments(Img m, double *out) {
   if (i >= m.n) return;
   double mean, var;
   phi_moments<DIM>(m, i, mean, var);
-  out[i] = mean;
-  out[m.n + i] = var;
+  out[2 * i] = mean;
+  out[2 * i + 1] = var;
 }
 
 // ---------------------------------------------------------------------------

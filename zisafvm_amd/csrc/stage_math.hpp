@@ -58,7 +58,7 @@ struct Img {
   const int32_t *fslot;  // 2 * ne: cell-face slot of each interior face's left / right cell
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
-  const double *phim;    // 2 * n: cell mean of phi and of (phi - mean)^2, or null
+  const double *phim;    // n pairs: cell mean of phi and of (phi - mean)^2 (one 16-byte load), or null
   const double *pot;     // potential tables of the cell's own points (pot_index), or null
   const double *frm;     // face frames of the cells' local faces (load_face_frame)
   const double *ffr;     // per interior face its frame and area, t1, area (2D) or n, t1, area (3D), x ne
@@ -816,7 +816,7 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // the same value.
   auto group_avg = [&](const EqC &tc, double &ar, double &aE) {
     if (PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m))) {
-      linear_isentropic_avg(tc, m.phim[i], m.phim[n + i], ar, aE);
+      linear_isentropic_avg(tc, m.phim[2 * i], m.phim[2 * i + 1], ar, aE);
       return;
     }
     const double vol_i = m.vol[i];
@@ -1116,8 +1116,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       }
       mb.lin = PH == PH_LIN || (PH == PH_FULL && linear_isentropic(m) && code == 0) ? 1 : 0;
       if (PH == PH_LIN || mb.lin) {
-        mb.pm = m.phim[j];
-        mb.pv = m.phim[n + j];
+        const PPair pv2 = reinterpret_cast<const PPair *>(m.phim)[j];
+        mb.pm = pv2.lo;
+        mb.pv = pv2.hi;
         return;
       }
       load_corners<DIM>(m, j, mb.w3);
