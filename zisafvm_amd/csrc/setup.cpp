@@ -1050,7 +1050,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   img.P.assign(size_t(s.m) * N, 0.0);
   img.rows.assign(size_t(s.nrows) * N, 0);
   img.twin.assign(size_t(std::max(s.nrows - s.rc, 1)) * N, int8_t(-1));
-  img.crn.assign(size_t(s.nfaces) * dim * N, 0.0);
+  img.crn.assign(size_t(dim == 2 ? 6 : 16) * N, 0.0);  // per cell, each vertex on 16 bytes
   img.xc.assign(size_t(dim) * N, 0.0);
   img.len.assign(N, 0.0);
   img.vol.assign(N, 0.0);
@@ -1081,7 +1081,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
           break;
         }
     for (int k = 0; k < nv; ++k)
-      for (int d = 0; d < dim; ++d) img.crn[size_t(k * dim + d) * N + ip] = g.c(i, k)[d];
+      for (int d = 0; d < dim; ++d) img.crn[size_t(ip) * (dim == 2 ? 6 : 16) + size_t(k) * (dim == 2 ? 2 : 4) + d] = g.c(i, k)[d];
     for (int d = 0; d < dim; ++d) img.xc[size_t(d) * N + ip] = g.ctr[i][d];
     img.len[ip] = g.len[i];
     img.vol[ip] = g.vol[i];

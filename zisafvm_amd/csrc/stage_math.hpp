@@ -110,6 +110,11 @@ ZFV_HD double sqr(double x) { return x * x; }
 ZFV_HD double dmin(double a, double b) { return a < b ? a : b; }
 ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
 
+// two adjacent doubles, one 16-byte load
+struct alignas(16) PPair {
+  double lo, hi;
+};
+
 // ---- gravity -----------------------------------------------------------------
 // PH selects the physics a kernel is compiled for: PH_LIGHT covers no or axial
 // gravity with no or isothermal equilibrium (exp only); PH_FULL adds radial
@@ -267,12 +272,31 @@ ZFV_HD void linear_isentropic_avg(const EqC &c, double phim, double phiv, double
 }
 
 // ---- geometry helpers ---------------------------------------------------------
+// Corners are stored per cell, vertex by vertex, each vertex on 16 bytes:
+// 2D (x, y), 3D (x, y, z, 0); corner_words doubles per cell
+template <int DIM>
+constexpr int corner_words() { return DIM == 2 ? 6 : 16; }
+
+// vertex c of cell i
+template <int DIM>
+ZFV_HD void load_corner(const Img &m, int64_t i, int c, double *w) {
+  const PPair *p = reinterpret_cast<const PPair *>(m.crn + i * corner_words<DIM>());
+  if (DIM == 2) {
+    const PPair a = p[c];
+    w[0] = a.lo;
+    w[1] = a.hi;
+    w[2] = 0.0;
+  } else {
+    const PPair a = p[2 * c], b = p[2 * c + 1];
+    w[0] = a.lo;
+    w[1] = a.hi;
+    w[2] = b.lo;
+  }
+}
+
 template <int DIM>
 ZFV_HD void load_corners(const Img &m, int64_t i, double (*v)[3]) {
-  for (int k = 0; k < DIM + 1; ++k) {
-    for (int d = 0; d < DIM; ++d) v[k][d] = m.crn[(k * DIM + d) * m.n + i];
-    if (DIM == 2) v[k][2] = 0.0;
-  }
+  for (int k = 0; k < DIM + 1; ++k) load_corner<DIM>(m, i, k, v[k]);
 }
 
 // vertices of local face f in the order of the face's left cell
@@ -290,11 +314,7 @@ ZFV_HD void face_vertices(uint32_t fperm, int f, const double (*v)[3], double (*
 // of selected from registers
 template <int DIM>
 ZFV_HD void load_face_vertices(const Img &m, int64_t i, uint32_t fperm, int f, double (*w)[3]) {
-  for (int r = 0; r < DIM; ++r) {
-    const int c = (fperm >> (2 * (f * 3 + r))) & 3u;
-    for (int d = 0; d < DIM; ++d) w[r][d] = m.crn[(c * DIM + d) * m.n + i];
-    if (DIM == 2) w[r][2] = 0.0;
-  }
+  for (int r = 0; r < DIM; ++r) load_corner<DIM>(m, i, int((fperm >> (2 * (f * 3 + r))) & 3u), w[r]);
 }
 
 // Face frame (src/zisa/math/face_factory.cpp:11-34): n, t1, t2 and the face
@@ -758,9 +778,6 @@ constexpr int wave_block_doubles() { return wave_lds_doubles<DIM, DOFC>() + (64 
 // the index of central coefficient a of row r of cell i in the pseudo-inverse
 // array (pairs a, a + 1 of one row and cell adjacent, one 16-byte load; an
 // odd last coefficient in its own block); the one-sided rows follow at nc rc n
-struct alignas(16) PPair {
-  double lo, hi;
-};
 ZFV_HD int64_t p_central(int nc, int rc, int64_t n, int a, int r, int64_t i) {
   const int ncp = nc / 2;
   if (a < 2 * ncp) return ((int64_t(a >> 1) * rc + r) * n + i) * 2 + (a & 1);
