@@ -1051,11 +1051,12 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   img.rows.assign(size_t(s.nrows) * N, 0);
   img.twin.assign(size_t(std::max(s.nrows - s.rc, 1)) * N, int8_t(-1));
   img.crn.assign(size_t(dim == 2 ? 6 : 16) * N, 0.0);  // per cell, each vertex on 16 bytes
-  img.xc.assign(size_t(dim) * N, 0.0);
+  const int xw = dim == 2 ? 2 : 4, mw = (s.nmom + 1) & ~1;  // per-cell centre and moment words
+  img.xc.assign(size_t(xw) * N, 0.0);
   img.len.assign(N, 0.0);
   img.vol.assign(N, 0.0);
   img.inr.assign(N, 0.0);
-  img.mom.assign(size_t(s.nmom) * N, 0.0);
+  img.mom.assign(size_t(std::max(mw, 1)) * N, 0.0);
   img.meta.assign(N, 0);
   img.cface.assign(size_t(s.nfaces) * N, 0);
   img.fperm.assign(N, 0u);
@@ -1082,12 +1083,12 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
         }
     for (int k = 0; k < nv; ++k)
       for (int d = 0; d < dim; ++d) img.crn[size_t(ip) * (dim == 2 ? 6 : 16) + size_t(k) * (dim == 2 ? 2 : 4) + d] = g.c(i, k)[d];
-    for (int d = 0; d < dim; ++d) img.xc[size_t(d) * N + ip] = g.ctr[i][d];
+    for (int d = 0; d < dim; ++d) img.xc[size_t(ip) * xw + d] = g.ctr[i][d];
     img.len[ip] = g.len[i];
     img.vol[ip] = g.vol[i];
     img.inr[ip] = g.inr[i];
     for (int a = 3; a < s.dofc; ++a)
-      img.mom[size_t(a - 3) * N + ip] = a < int(g.mom[i].size()) ? g.mom[i][a] : 0.0;
+      img.mom[size_t(ip) * mw + (a - 3)] = a < int(g.mom[i].size()) ? g.mom[i][a] : 0.0;
     int32_t meta = cs[i].khigh & META_KHIGH_MASK;
     if (cs[i].first_order) meta |= META_FIRST_ORDER;
     if (g.ghost[i]) meta |= META_GHOST;

@@ -294,6 +294,37 @@ ZFV_HD void load_corner(const Img &m, int64_t i, int c, double *w) {
   }
 }
 
+// The centre (2D x y, 3D x y z 0) and the moments (indices 3 .. DOFC-1,
+// padded to an even count) are stored per cell as well, in 16-byte pieces
+template <int DIM>
+constexpr int centre_words() { return DIM == 2 ? 2 : 4; }
+template <int DOFC>
+constexpr int moment_words() { return ((DOFC > 3 ? DOFC - 3 : 0) + 1) & ~1; }
+
+template <int DIM>
+ZFV_HD void load_centre(const Img &m, int64_t i, double *xc) {
+  const PPair *p = reinterpret_cast<const PPair *>(m.xc + i * centre_words<DIM>());
+  const PPair a = p[0];
+  xc[0] = a.lo;
+  xc[1] = a.hi;
+  xc[2] = DIM == 3 ? p[1].lo : 0.0;
+}
+
+// cm[alpha] for alpha < DOFC: zero below 3, the cell's moments from 3 on
+template <int DOFC>
+ZFV_HD void load_moments(const Img &m, int64_t i, double *cm) {
+  constexpr int MW = moment_words<DOFC>();
+  for (int al = 0; al < DOFC && al < 3; ++al) cm[al] = 0.0;
+  if (MW > 0) {
+    const PPair *p = reinterpret_cast<const PPair *>(m.mom + i * MW);
+    for (int k = 0; k < MW / 2; ++k) {
+      const PPair v = p[k];
+      if (3 + 2 * k < DOFC) cm[3 + 2 * k] = v.lo;
+      if (4 + 2 * k < DOFC) cm[4 + 2 * k] = v.hi;
+    }
+  }
+}
+
 template <int DIM>
 ZFV_HD void load_corners(const Img &m, int64_t i, double (*v)[3]) {
   for (int k = 0; k < DIM + 1; ++k) load_corner<DIM>(m, i, k, v[k]);
@@ -650,8 +681,9 @@ ZFV_HD void store_source_moments(const Img &m, int64_t i, double *tab) {
   double vtx[DIM + 1][3];
   load_corners<DIM>(m, i, vtx);
   double cm[DOFC];
-  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
-  const double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
+  load_moments<DOFC>(m, i, cm);
+  double xc[3];
+  load_centre<DIM>(m, i, xc);
   const double inv_len = 1.0 / m.len[i];
   const double vol_i = m.vol[i];
   double acc[DOFC][DIM];
@@ -1349,8 +1381,9 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
 
   // moments: zero for alpha < 3
   double cm[DOFC];
-  for (int al = 0; al < DOFC; ++al) cm[al] = al < 3 ? 0.0 : m.mom[(al - 3) * n + i];
-  double xc[3] = {m.xc[i], m.xc[n + i], DIM == 3 ? m.xc[2 * n + i] : 0.0};
+  load_moments<DOFC>(m, i, cm);
+  double xc[3];
+  load_centre<DIM>(m, i, xc);
   const double len = m.len[i];
   const uint32_t fperm = m.fperm[i];
 
