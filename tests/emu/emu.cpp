@@ -353,7 +353,7 @@ int zfv_emu_validate(void *h, char *msg, int cap) {
   for (int q = 1; q < 16; ++q)
     if (im.shift[3 * q] != 0.0 || im.shift[3 * q + 1] != 0.0 || im.shift[3 * q + 2] != 0.0) n_codes = q + 1;
   if (int64_t(im.rows.size()) != int64_t(s.nrows) * n) bad("rows size", -1);
-  if (int64_t(im.cface.size()) != int64_t(s.nfaces) * n) bad("cface size", -1);
+  if (int64_t(im.cface.size()) != int64_t(4) * n) bad("cface size", -1);
   std::vector<int> face_uses(size_t(ne > 0 ? ne : 1), 0);
   for (int64_t i = 0; i < n && err.empty(); ++i) {
     for (int r = 0; r < s.nrows; ++r) {
@@ -372,7 +372,7 @@ int zfv_emu_validate(void *h, char *msg, int cap) {
     const int32_t meta = im.meta[i];
     if ((meta & 7) > s.nb) bad("CWENO stencil index out of range", i);
     for (int f = 0; f < s.nfaces; ++f) {
-      const int32_t cf = im.cface[size_t(f) * n + i];
+      const int32_t cf = im.cface[size_t(i) * 4 + f];
       if ((cf & 2) == 0) {
         const int64_t fe = cf >> 2;
         if (fe < 0 || fe >= ne) bad("face id out of range", i);

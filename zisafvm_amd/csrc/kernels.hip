@@ -1429,7 +1429,7 @@ int zfv_estimate_device_bytes(const zfv_mesh *mesh, const zfv_params *params, in
   const double nv = s.nvars;
   const double frame = s.dim == 2 ? frame_words<2>() : frame_words<3>();
   double per_cell = 8.0 * s.m + 4.0 * s.nrows + 1.0 * (s.nrows - s.rc) +
-                    8.0 * ((s.dim == 2 ? 6 : 16) + s.dim + 3 + s.nmom) + 4.0 * (2 + s.nfaces) +
+                    8.0 * ((s.dim == 2 ? 6 : 16) + s.dim + 3 + s.nmom) + 4.0 * (2 + 4) +
                     8.0 * s.nfaces * frame +    // face frames
                     8.0 * nv * (1 + 1 + 3);     // U, X, K[3]
   if (params->gravity != ZFV_GRAVITY_NONE) {

@@ -1058,7 +1058,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
   img.inr.assign(N, 0.0);
   img.mom.assign(size_t(std::max(mw, 1)) * N, 0.0);
   img.meta.assign(N, 0);
-  img.cface.assign(size_t(s.nfaces) * N, 0);
+  img.cface.assign(size_t(4) * N, 0);  // per cell 4 words (2D: the fourth unused)
   img.fperm.assign(N, 0u);
   img.ghost.assign(N, 0);
 #pragma omp parallel for schedule(static)
@@ -1117,7 +1117,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
           }
         }
       }
-      img.cface[size_t(k) * N + ip] = cf;
+      img.cface[size_t(ip) * 4 + k] = cf;
       for (int r = 0; r < dim; ++r) fp |= uint32_t(lv[r]) << (2 * (k * 3 + r));
     }
     img.fperm[ip] = fp;
@@ -1127,7 +1127,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
 #pragma omp parallel for schedule(static)
   for (int ip = 0; ip < n; ++ip)
     for (int k = 0; k < nv; ++k) {
-      const int32_t cf = img.cface[size_t(k) * N + ip];
+      const int32_t cf = img.cface[size_t(ip) * 4 + k];
       if (cf & CFACE_EXTERIOR) continue;
       img.fslot[size_t(cf & CFACE_SIDE_R) * size_t(img.ne) + size_t(cf >> 2)] = ip * nv + k;
     }

@@ -56,7 +56,7 @@ struct HostImage {
   std::vector<double> len, vol, inr;
   std::vector<double> mom;       // nmom * n
   std::vector<int32_t> meta;     // n
-  std::vector<int32_t> cface;    // nfaces * n
+  std::vector<int32_t> cface;    // 4 * n: per cell its face words (2D: the fourth unused)
   // the cell-face slots (i * nfaces + k, product numbering) of each interior
   // face's left (side 0, first ne entries) and right (side 1) cell: where the
   // reconstruction leaves the face's point values

@@ -54,7 +54,7 @@ struct Img {
   const double *inr;
   const double *mom;
   const int32_t *meta;
-  const int32_t *cface;
+  const int32_t *cface;  // per cell its face words, 4 int32 (one 16-byte load; 2D's fourth unused)
   const int32_t *fslot;  // 2 * ne: cell-face slot of each interior face's left / right cell
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
@@ -114,6 +114,12 @@ ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
 struct alignas(16) PPair {
   double lo, hi;
 };
+
+// a cell's four face words (face id << 2 | exterior << 1 | side), one load
+struct alignas(16) FaceWords {
+  int32_t w[4];
+};
+ZFV_HD FaceWords load_face_words(const Img &m, int64_t i) { return reinterpret_cast<const FaceWords *>(m.cface)[i]; }
 
 // ---- gravity -----------------------------------------------------------------
 // PH selects the physics a kernel is compiled for: PH_LIGHT covers no or axial
@@ -1412,9 +1418,10 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
   // extrapolation does.  (Isentropic only: never in the light physics set.)
   const bool eq_flux_bc = PH != PH_LIGHT && m.flux_bc == 2 && gravity_src;
 
+  const FaceWords cfw = load_face_words(m, i);
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    const int32_t cf = m.cface[f * n + i];
+    const int32_t cf = cfw.w[f];
     double w3[3][3];
     load_face_vertices<DIM>(m, i, fperm, f, w3);
     double nn[3], t1[3], t2[3], area;
@@ -1627,9 +1634,10 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
   const int64_t n = m.n;
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const FaceWords cfw = load_face_words(m, i);
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    const int32_t cf = m.cface[f * n + i];
+    const int32_t cf = cfw.w[f];
     const bool exterior = (cf & 2) != 0;
     // ghost-pair faces (cf == 3) carry neither a flux nor the flux BC; the
     // equilibrium flux BC (flux_bc 2) is part of the source term
