@@ -149,7 +149,7 @@ ZFV_HD void kb_update_thread(DevInst *insts, BlockRef b, int t, int stage) {
   const double c2 = 2.0 / 3;
   const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
   for (int v = 0; v < NV; ++v) {
-    d.K[(int64_t(stage) * NV + v) * n + i] = L[v];
+    if (stage < 2) d.K[(int64_t(stage) * NV + v) * n + i] = L[v];  // the last stage's is read by no later one
     double dudt = 0.0;
     if (stage == 0) {
       dudt += c0 * L[v];

@@ -170,7 +170,9 @@ __global__ __launch_bounds__(256) void k_update(Img m, StageCoeffs sc, int stage
   }
   double L[NV];
   cell_rate<DIM>(m, i, Ustage, G, src, L);
-  for (int v = 0; v < NV; ++v) K[(int64_t(stage) * NV + v) * n + i] = L[v];
+  // the last stage's tendency enters only its own sum (no later stage reads it)
+  if (sc.write_rate_only || stage + 1 < 3)
+    for (int v = 0; v < NV; ++v) K[(int64_t(stage) * NV + v) * n + i] = L[v];
   if (sc.write_rate_only) return;
   const bool ghost = (m.meta[i] & 16) != 0 && m.steady != nullptr;
   for (int v = 0; v < NV; ++v) {
