@@ -1064,11 +1064,18 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
 #pragma omp parallel for schedule(static)
   for (int ip = 0; ip < n; ++ip) {
     const int i = img.perm[ip];
-    // central coefficient a of row r (flat a rc + r) at p_central, the
-    // one-sided rows flat
+    // central coefficient a of row r (flat a rc + r) at p_central, one-sided
+    // coefficient a of row rr of stencil kb (flat kb nbc rb + a rb + rr after
+    // the central ones) at p_onesided
+    const int nc = s.dofc - 1, nbc = s.dofb - 1;
     for (int m = 0; m < s.m; ++m) {
-      const int nc = s.dofc - 1;
-      const size_t at = m < nc * s.rc ? size_t(p_central(nc, s.rc, n, m / s.rc, m % s.rc, ip)) : size_t(m) * N + ip;
+      size_t at;
+      if (m < nc * s.rc) {
+        at = size_t(p_central(nc, s.rc, n, m / s.rc, m % s.rc, ip));
+      } else {
+        const int o = m - nc * s.rc, kb = o / (nbc * s.rb), a = (o % (nbc * s.rb)) / s.rb, rr = o % s.rb;
+        at = size_t(p_onesided(nc, s.rc, nbc, s.nb, s.rb, n, kb, a, rr, ip));
+      }
       img.P[at] = Pc[size_t(i) * s.m + m];
     }
     for (int r = 0; r < s.nrows; ++r) {

@@ -816,6 +816,18 @@ constexpr int wave_block_doubles() { return wave_lds_doubles<DIM, DOFC>() + (64 
 // the index of central coefficient a of row r of cell i in the pseudo-inverse
 // array (pairs a, a + 1 of one row and cell adjacent, one 16-byte load; an
 // odd last coefficient in its own block); the one-sided rows follow at nc rc n
+// the index of coefficient a of row rr of one-sided stencil kb (nbc
+// coefficients, rb rows each): after the central block, pairs a, a + 1 of a
+// row adjacent as for the central rows, an odd last coefficient in its own
+// block
+ZFV_HD int64_t p_onesided(int nc, int rc, int nbc, int nb, int rb, int64_t n, int kb, int a, int rr, int64_t i) {
+  const int64_t base = int64_t(nc) * rc * n;
+  const int npair = nbc / 2;
+  const int64_t row = int64_t(kb) * rb + rr;
+  if (a < 2 * npair) return base + ((row * npair + (a >> 1)) * n + i) * 2 + (a & 1);
+  return base + int64_t(2 * npair) * nb * rb * n + row * n + i;
+}
+
 ZFV_HD int64_t p_central(int nc, int rc, int64_t n, int a, int r, int64_t i) {
   const int ncp = nc / 2;
   if (a < 2 * ncp) return ((int64_t(a >> 1) * rc + r) * n + i) * 2 + (a & 1);
@@ -1255,8 +1267,17 @@ ZFV_HD void reconstruct_lane(const Img &m, int64_t i, const int g, X &xch, const
       } else {
         const int kb = (my_r - RC) / RB, rr = (my_r - RC) - kb * RB;
 #pragma unroll
-        for (int a = 0; a < NC; ++a)
-          pr[a] = (mine && a < NBC) ? m.P[(NC * RC + kb * NBC * RB + a * RB + rr) * n + i] : 0.0;
+        for (int a = 0; a < NC; ++a) pr[a] = 0.0;
+        if (mine) {
+          const PPair *pp = reinterpret_cast<const PPair *>(m.P);
+#pragma unroll
+          for (int p = 0; p < NBC / 2; ++p) {
+            const PPair v = pp[p_onesided(NC, RC, NBC, NB, RB, n, kb, 2 * p, rr, i) >> 1];
+            pr[2 * p] = v.lo;
+            pr[2 * p + 1] = v.hi;
+          }
+          if (NBC & 1) pr[NBC - 1] = m.P[p_onesided(NC, RC, NBC, NB, RB, n, kb, NBC - 1, rr, i)];
+        }
       }
       xch.sync();
 #pragma unroll
