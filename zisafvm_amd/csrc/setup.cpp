@@ -1136,7 +1136,7 @@ void build_image(const zfv_mesh &mesh, const zfv_params &p, HostImage &img) {
     for (int k = 0; k < nv; ++k) {
       const int32_t cf = img.cface[size_t(ip) * 4 + k];
       if (cf & CFACE_EXTERIOR) continue;
-      img.fslot[size_t(cf & CFACE_SIDE_R) * size_t(img.ne) + size_t(cf >> 2)] = ip * nv + k;
+      img.fslot[size_t(cf >> 2) * 2 + size_t(cf & CFACE_SIDE_R)] = ip * nv + k;
     }
 
   // normalized rules for the kernels

@@ -60,7 +60,7 @@ struct HostImage {
   // the cell-face slots (i * nfaces + k, product numbering) of each interior
   // face's left (side 0, first ne entries) and right (side 1) cell: where the
   // reconstruction leaves the face's point values
-  std::vector<int32_t> fslot;    // 2 * ne
+  std::vector<int32_t> fslot;    // ne pairs (left slot, right slot)
   std::vector<uint32_t> fperm;   // n: own-corner index of the face's left-cell vertex order
   std::vector<int32_t> perm;     // product index -> caller index
   std::vector<int32_t> iperm;    // caller index -> product index

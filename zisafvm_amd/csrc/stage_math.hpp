@@ -55,7 +55,7 @@ struct Img {
   const double *mom;
   const int32_t *meta;
   const int32_t *cface;  // per cell its face words, 4 int32 (one 16-byte load; 2D's fourth unused)
-  const int32_t *fslot;  // 2 * ne: cell-face slot of each interior face's left / right cell
+  const int32_t *fslot;  // ne pairs: cell-face slots of each interior face's left and right cell
   const uint32_t *fperm;
   const double *steady;  // nvars * n (ghost cells only are read)
   const double *phim;    // n pairs: cell mean of phi and of (phi - mean)^2 (one 16-byte load), or null
@@ -114,6 +114,15 @@ ZFV_HD double dmax(double a, double b) { return a > b ? a : b; }
 struct alignas(16) PPair {
   double lo, hi;
 };
+
+// an interior face's two cell-face slots (left, right), one 8-byte load
+struct alignas(8) SlotPair {
+  int32_t l, r;
+};
+
+// Face fluxes G: 2D per face its 4 values (two 16-byte pieces), 3D one
+// plane of ne values per variable (an odd count would need a pad word per
+// face; measured slower)
 
 // a cell's four face words (face id << 2 | exterior << 1 | side), one load
 struct alignas(16) FaceWords {
@@ -469,7 +478,7 @@ ZFV_HD void load_face_frame(const Img &m, int64_t i, int f, double *n, double *t
 template <int DIM>
 ZFV_HD void store_face_table(const Img &m, int64_t e, double *tab) {
   constexpr int NF = DIM + 1;
-  const int64_t s = m.fslot[e];
+  const int64_t s = m.fslot[2 * e];
   double n[3], t1[3], t2[3], area;
   load_face_frame<DIM>(m, s / NF, int(s % NF), n, t1, t2, area);
   if (DIM == 2) {
@@ -1604,8 +1613,9 @@ ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, d
   // each slot, momentum in the global frame), in 16-byte loads
   constexpr int SQ = slot_doubles<DIM, Q>();
   double bL[SQ], bR[SQ];
-  const PPair *pL = reinterpret_cast<const PPair *>(fpv + int64_t(m.fslot[e]) * SQ);
-  const PPair *pR = reinterpret_cast<const PPair *>(fpv + int64_t(m.fslot[ne + e]) * SQ);
+  const SlotPair sl = reinterpret_cast<const SlotPair *>(m.fslot)[e];
+  const PPair *pL = reinterpret_cast<const PPair *>(fpv + int64_t(sl.l) * SQ);
+  const PPair *pR = reinterpret_cast<const PPair *>(fpv + int64_t(sl.r) * SQ);
 #pragma unroll
   for (int k = 0; k < SQ / 2; ++k) {
     const PPair l = pL[k], r = pR[k];
@@ -1642,7 +1652,13 @@ ZFV_HD void face_flux(const Img &m, int64_t e, const double *__restrict__ fpv, d
   // face, which the left cell loses and the right cell gains (cell_rate)
   double wv[NV];
   world_flux<DIM>(acc, nn, t1, t2, wv);
-  for (int v = 0; v < NV; ++v) G[v * ne + e] = area * wv[v];
+  if (DIM == 2) {
+    PPair *gp = reinterpret_cast<PPair *>(G + e * NV);
+    gp[0] = PPair{area * wv[0], area * wv[1]};
+    gp[1] = PPair{area * wv[2], area * wv[NV - 1]};
+  } else {
+    for (int v = 0; v < NV; ++v) G[v * ne + e] = area * wv[v];
+  }
 }
 
 // ---- the per-cell rate of change -------------------------------------------------
@@ -1667,7 +1683,18 @@ ZFV_HD void cell_rate(const Img &m, int64_t i, const double *__restrict__ U, con
       // the face kernel's flux through the face (global frame, times area)
       const int64_t e = cf >> 2;
       const double sgn = (cf & 1) ? 1.0 : -1.0;
-      for (int v = 0; v < NV; ++v) acc[v] += sgn * G[v * m.ne + e];
+      double g[NV];
+      if (DIM == 2) {
+        const PPair *gp = reinterpret_cast<const PPair *>(G + e * NV);
+        const PPair g0 = gp[0], g1 = gp[1];
+        g[0] = g0.lo;
+        g[1] = g0.hi;
+        g[2] = g1.lo;
+        g[NV - 1] = g1.hi;
+      } else {
+        for (int v = 0; v < NV; ++v) g[v] = G[v * m.ne + e];
+      }
+      for (int v = 0; v < NV; ++v) acc[v] += sgn * g[v];
       continue;
     }
     // FluxBC: physical flux of the cell average in the face frame
